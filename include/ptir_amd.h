@@ -1,0 +1,171 @@
+/*
+ * ptir_amd.h -- C ABI of the MI355X (gfx950) differentiable Gaussian path tracer.
+ *
+ * Drop-in boundary for the native surface of the reference's `threedgptir_tracer`
+ * plugin.  Every entry point takes plain device pointers, sizes and a HIP stream
+ * (passed as `void*`, NULL = default stream) and returns a ptir_status.  Nothing in
+ * this header depends on torch.  All device buffers are float32, contiguous, row-major,
+ * with the reference's shapes.  The library never synchronises the stream; it only
+ * enqueues work on it.  A tracer handle is not re-entrant: calls on one handle must be
+ * serialised by the caller (as in the reference, whose params buffer is shared).
+ */
+#ifndef PTIR_AMD_H
+#define PTIR_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum ptir_status {
+    PTIR_OK = 0,
+    PTIR_ERR_INVALID_ARGUMENT = 1,   /* bad shape, size, null pointer or option */
+    PTIR_ERR_UNSUPPORTED = 2,        /* option the build does not implement */
+    PTIR_ERR_NOT_BUILT = 3,          /* trace before build_bvh, or N changed since */
+    PTIR_ERR_HIP = 4,                /* a HIP runtime call failed (see ptir_last_error) */
+    PTIR_ERR_BVH_DEPTH = 5,          /* hierarchy deeper than the traversal stack */
+    PTIR_ERR_OUT_OF_MEMORY = 6
+} ptir_status;
+
+typedef struct ptir_tracer ptir_tracer;
+
+/* Constructor flags.  Replaces OptixTracer::OptixTracer
+ * (threedgptir_tracer/src/optixTracer.cpp:274-343, bindings.cpp:35): the path, cuda_path,
+ * pipeline and primitive strings of the reference select OptiX programs and have no
+ * counterpart; only "reference"/"referenceBwd" pipelines on "instances" primitives exist. */
+typedef struct ptir_options {
+    float kernel_degree;          /* generalized Gaussian degree: 0,1,2,3,4,5,8 (PTIR: 4) */
+    float min_response;           /* particle_kernel_min_response (PTIR: 0.0113) */
+    int32_t density_clamping;     /* particle_kernel_density_clamping (PTIR: 1) */
+    int32_t sph_max_degree;       /* particle_radiance_sph_degree (SH storage degree, <= 3) */
+    int32_t enable_normals;
+    int32_t enable_hitcounts;
+    int32_t enable_mis;
+    int32_t enable_metallic;
+    int32_t visualize_lights;     /* must be 0 (light visualisation is not built) */
+    int32_t kbuffer;              /* hits collected per traversal chunk: 8, 16 (reference) or 32 */
+} ptir_options;
+
+/* Scene inputs of one trace call (OptixTracer::trace argument list,
+ * threedgptir_tracer/src/optixTracer.cpp:909-929).  Light tensors other than the
+ * environment must be empty (num_lights == 0, num_light_entries == 0, num_mesh_lights == 0). */
+typedef struct ptir_scene {
+    uint32_t num_gaussians;
+    const float* particle_density;    /* [N,12] pos3, density, quat(wxyz)4, scale3, pad */
+    const float* particle_material;   /* [N,5]  albedo3, roughness, metallic */
+    const float* particle_radiance;   /* [N, sh_stride] SH, coefficient-major, RGB inner */
+    int32_t sh_stride;                /* floats per Gaussian in particle_radiance */
+    const float* shading_normal;      /* [N,3] */
+    const float* environment;         /* [env_h, env_w, 4] or NULL */
+    int32_t env_h, env_w;
+    const float* env_table;           /* [3, tab_h, tab_w] (prob, alias, pdf) or NULL */
+    int32_t tab_h, tab_w;
+    uint32_t num_lights;              /* lights [L,9]; must be 0 */
+    uint32_t num_light_entries;       /* lightAliasTable [5,E]; must be 0 */
+    uint32_t num_mesh_lights;         /* meshLights [M,8]; must be 0 */
+} ptir_scene;
+
+/* Launch description.  rays are [B,H,W,3]; ray_to_world is a DEVICE pointer to a
+ * [*,4,4] float32 matrix of which only the first 12 floats are used (the reference reads
+ * them from rayToWorld[0] on the host, optixTracer.cpp:962; here the kernel reads them on
+ * the device, so no host synchronisation happens and the call can be graph-captured). */
+typedef struct ptir_launch {
+    uint32_t frame;
+    const float* ray_to_world;
+    const float* ray_ori;
+    const float* ray_dir;
+    int32_t B, H, W;
+    uint32_t render_opts;             /* 0 = NONE, 1 = INDIRECT */
+    int32_t sph_degree;               /* evaluated SH degree (n_active_features) */
+    float min_transmittance;
+    uint32_t max_bounces;             /* path vertices incl. the primary hit */
+} ptir_launch;
+
+/* Forward outputs, all [B,H,W,C] float32 unless noted (optixTracer.cpp:931-944).
+ * The library writes every element: G-buffer outputs always, pbr/light/components zero
+ * where the primary interaction is invalid.  visibility is [N] float32 whose bits are
+ * set to the int32 value 1 for every Gaussian accepted on a traced path, and must be
+ * zeroed by the caller before the call (the reference allocates it with torch::zeros). */
+typedef struct ptir_fwd_outputs {
+    float* radiance;       /* [..,3] */
+    float* density;        /* [..,1] */
+    float* hit_distance;   /* [..,2] integrated depth, last processed hit distance */
+    float* hit_distance2;  /* [..,1] */
+    float* distortion;     /* [..,1] */
+    float* normal;         /* [..,3] */
+    float* shading_normal; /* [..,3] */
+    float* material;       /* [..,5] */
+    float* hits_count;     /* [..,1] */
+    float* visibility;     /* [N] */
+    float* pbr;            /* [..,3] */
+    float* light;          /* [..,3] */
+    float* pbr_components; /* [..,2,3] direct, indirect */
+} ptir_fwd_outputs;
+
+/* Backward inputs: forward rayPbr/rayLight and the incoming gradients.  The reference's
+ * backward reads only these among the 10 forward outputs / 10 output gradients it is
+ * passed (referenceBwdOptix.cu:38-51).  light_grad and material_grad may be NULL (zero). */
+typedef struct ptir_bwd_inputs {
+    const float* pbr;            /* [..,3] */
+    const float* light;          /* [..,3] */
+    const float* pbr_grad;       /* [..,3] */
+    const float* light_grad;     /* [..,3] or NULL */
+    const float* material_grad;  /* [..,5] or NULL */
+} ptir_bwd_inputs;
+
+/* Backward outputs (OptixTracer::traceBwd, optixTracer.cpp:1058-1063).  The library
+ * zeroes and writes every element.  In the reference semantics only material and env
+ * gradients are non-zero; density/SH/shading-normal gradients are written as zeros. */
+typedef struct ptir_bwd_outputs {
+    float* density_grad;          /* [N,12] */
+    float* material_grad;         /* [N,5] */
+    float* radiance_grad;         /* [N, sh_stride] */
+    float* shading_normal_grad;   /* [N,3] */
+    float* environment_grad;      /* [env_h, env_w, 4] or NULL when there is no env */
+} ptir_bwd_outputs;
+
+/* Work counters of one call (optional): units of the algorithmic byte model. */
+typedef struct ptir_counters {
+    uint64_t paths, segments, candidates, accepted, accepted_primary, shadow_candidates, env_lookups;
+    uint64_t mat_atomics, env_atomics;
+} ptir_counters;
+
+/* OptixTracer::OptixTracer (optixTracer.cpp:274-343).  Allocates no device memory. */
+ptir_status ptir_create(const ptir_options* options, ptir_tracer** out);
+
+/* ~OptixTracer (optixTracer.cpp:345-371).  Frees the handle's device buffers. */
+ptir_status ptir_destroy(ptir_tracer* tracer);
+
+/* OptixTracer::buildBVH (optixTracer.cpp:633-907): pos [N,3], rot [N,4] (unit wxyz),
+ * scl [N,3] (activated), dns [N] (activated), all device pointers.  rebuild = 0 refits
+ * the existing hierarchy (topology kept, boxes recomputed); refitting with a different N
+ * is an error here (the reference prints and rebuilds).  allow_update is accepted for
+ * parity and has no effect: every hierarchy can be refitted. */
+ptir_status ptir_build_bvh(ptir_tracer* tracer, const float* pos, const float* rot, const float* scl, const float* dns,
+                           uint32_t num_gaussians, uint32_t rebuild, int32_t allow_update, void* stream);
+
+/* OptixTracer::trace (optixTracer.cpp:909-1013).  counters may be NULL; when not NULL
+ * the kernels count work units into device memory and the call copies them back
+ * synchronously (use only for measurement). */
+ptir_status ptir_trace(ptir_tracer* tracer, const ptir_scene* scene, const ptir_launch* launch,
+                       const ptir_fwd_outputs* out, ptir_counters* counters, void* stream);
+
+/* OptixTracer::traceBwd (optixTracer.cpp:1015-1145), path-replay backward. */
+ptir_status ptir_trace_bwd(ptir_tracer* tracer, const ptir_scene* scene, const ptir_launch* launch,
+                           const ptir_bwd_inputs* in, const ptir_bwd_outputs* out, ptir_counters* counters,
+                           void* stream);
+
+/* Human-readable text of the last error on this handle (or of the last ptir_create
+ * failure when tracer is NULL).  Valid until the next call on the handle. */
+const char* ptir_last_error(const ptir_tracer* tracer);
+
+/* Library version string. */
+const char* ptir_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PTIR_AMD_H */

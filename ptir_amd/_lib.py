@@ -1,0 +1,110 @@
+"""ctypes binding of libptir_amd.so (the C ABI declared in include/ptir_amd.h).
+
+The library is built in-tree by ``__graft_entry__.build()`` (ptir_amd/csrc/Makefile).
+Loading fails loudly when it is missing: there is no CPU or eager fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libptir_amd.so")
+
+PTIR_STATUS = {
+    0: "PTIR_OK",
+    1: "PTIR_ERR_INVALID_ARGUMENT",
+    2: "PTIR_ERR_UNSUPPORTED",
+    3: "PTIR_ERR_NOT_BUILT",
+    4: "PTIR_ERR_HIP",
+    5: "PTIR_ERR_BVH_DEPTH",
+    6: "PTIR_ERR_OUT_OF_MEMORY",
+}
+
+EXPORTED = ["ptir_create", "ptir_destroy", "ptir_build_bvh", "ptir_trace", "ptir_trace_bwd", "ptir_last_error",
+            "ptir_version"]
+
+
+class Options(C.Structure):
+    _fields_ = [("kernel_degree", C.c_float), ("min_response", C.c_float), ("density_clamping", C.c_int32),
+                ("sph_max_degree", C.c_int32), ("enable_normals", C.c_int32), ("enable_hitcounts", C.c_int32),
+                ("enable_mis", C.c_int32), ("enable_metallic", C.c_int32), ("visualize_lights", C.c_int32),
+                ("kbuffer", C.c_int32)]
+
+
+class Scene(C.Structure):
+    _fields_ = [("num_gaussians", C.c_uint32), ("particle_density", C.c_void_p), ("particle_material", C.c_void_p),
+                ("particle_radiance", C.c_void_p), ("sh_stride", C.c_int32), ("shading_normal", C.c_void_p),
+                ("environment", C.c_void_p), ("env_h", C.c_int32), ("env_w", C.c_int32),
+                ("env_table", C.c_void_p), ("tab_h", C.c_int32), ("tab_w", C.c_int32),
+                ("num_lights", C.c_uint32), ("num_light_entries", C.c_uint32), ("num_mesh_lights", C.c_uint32)]
+
+
+class Launch(C.Structure):
+    _fields_ = [("frame", C.c_uint32), ("ray_to_world", C.c_void_p), ("ray_ori", C.c_void_p), ("ray_dir", C.c_void_p),
+                ("B", C.c_int32), ("H", C.c_int32), ("W", C.c_int32), ("render_opts", C.c_uint32),
+                ("sph_degree", C.c_int32), ("min_transmittance", C.c_float), ("max_bounces", C.c_uint32)]
+
+
+class FwdOutputs(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("radiance", "density", "hit_distance", "hit_distance2", "distortion",
+                                          "normal", "shading_normal", "material", "hits_count", "visibility", "pbr",
+                                          "light", "pbr_components")]
+
+
+class BwdInputs(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("pbr", "light", "pbr_grad", "light_grad", "material_grad")]
+
+
+class BwdOutputs(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("density_grad", "material_grad", "radiance_grad", "shading_normal_grad",
+                                          "environment_grad")]
+
+
+class Counters(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in ("paths", "segments", "candidates", "accepted", "accepted_primary",
+                                          "shadow_candidates", "env_lookups", "mat_atomics", "env_atomics")]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libptir_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                               "(hipcc --offload-arch=gfx950); there is no fallback path")
+        L = C.CDLL(LIB_PATH)
+        L.ptir_create.argtypes = [C.POINTER(Options), C.POINTER(C.c_void_p)]
+        L.ptir_destroy.argtypes = [C.c_void_p]
+        L.ptir_build_bvh.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                     C.c_uint32, C.c_int32, C.c_void_p]
+        L.ptir_trace.argtypes = [C.c_void_p, C.POINTER(Scene), C.POINTER(Launch), C.POINTER(FwdOutputs),
+                                 C.POINTER(Counters), C.c_void_p]
+        L.ptir_trace_bwd.argtypes = [C.c_void_p, C.POINTER(Scene), C.POINTER(Launch), C.POINTER(BwdInputs),
+                                     C.POINTER(BwdOutputs), C.POINTER(Counters), C.c_void_p]
+        L.ptir_last_error.argtypes = [C.c_void_p]
+        L.ptir_last_error.restype = C.c_char_p
+        L.ptir_version.restype = C.c_char_p
+        for name in ("ptir_create", "ptir_destroy", "ptir_build_bvh", "ptir_trace", "ptir_trace_bwd"):
+            getattr(L, name).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+class PtirError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{PTIR_STATUS.get(status, status)}: {message}")
+        self.status = status
+
+
+def check(status: int, handle) -> None:
+    if status != 0:
+        msg = lib().ptir_last_error(handle)
+        raise PtirError(status, msg.decode() if msg else "")

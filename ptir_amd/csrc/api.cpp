@@ -1,0 +1,271 @@
+// C ABI of the tracer (declared in include/ptir_amd.h).
+#include "../../include/ptir_amd.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "internal.hpp"
+
+struct ptir_tracer {
+    ptir_options opts;
+    ptir::BvhBuffers bvh;
+    bool built = false;
+    std::string err;
+};
+
+namespace {
+
+thread_local std::string g_create_error;
+
+ptir_status fail(ptir_tracer* t, ptir_status st, const std::string& msg) {
+    if (t) t->err = msg;
+    else g_create_error = msg;
+    return st;
+}
+
+ptir_status hip_fail(ptir_tracer* t, hipError_t e, const char* where) {
+    return fail(t, PTIR_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+ptir::KernelOpts kopts(const ptir_options& o) {
+    return ptir::KernelOpts{int(o.kernel_degree), o.min_response, o.density_clamping};
+}
+
+bool valid_degree(float d) {
+    const int i = int(d);
+    return float(i) == d && (i == 0 || i == 1 || i == 2 || i == 3 || i == 4 || i == 5 || i == 8);
+}
+
+ptir_status check_scene(ptir_tracer* t, const ptir_scene* sc, const ptir_launch* la) {
+    if (!sc || !la) return fail(t, PTIR_ERR_INVALID_ARGUMENT, "null scene or launch");
+    if (!t->built) return fail(t, PTIR_ERR_NOT_BUILT, "trace called before build_bvh");
+    if (sc->num_gaussians != t->bvh.n)
+        return fail(t, PTIR_ERR_NOT_BUILT, "number of Gaussians differs from the last build_bvh (" +
+                                               std::to_string(sc->num_gaussians) + " vs " + std::to_string(t->bvh.n) + ")");
+    if (sc->num_lights || sc->num_light_entries || sc->num_mesh_lights)
+        return fail(t, PTIR_ERR_UNSUPPORTED, "point, sphere and mesh lights are not supported; pass empty light tensors");
+    if (sc->num_gaussians && (!sc->particle_density || !sc->particle_material || !sc->particle_radiance))
+        return fail(t, PTIR_ERR_INVALID_ARGUMENT, "null particle tensor");
+    if (la->B < 0 || la->H < 0 || la->W < 0) return fail(t, PTIR_ERR_INVALID_ARGUMENT, "negative ray shape");
+    const long long paths = (long long)la->B * la->H * la->W;
+    if (paths > 0 && (!la->ray_ori || !la->ray_dir || !la->ray_to_world))
+        return fail(t, PTIR_ERR_INVALID_ARGUMENT, "null ray tensor");
+    if (paths > (1ll << 31) - 1) return fail(t, PTIR_ERR_INVALID_ARGUMENT, "more than 2^31-1 paths in one launch");
+    if (la->render_opts > 1u) return fail(t, PTIR_ERR_INVALID_ARGUMENT, "render_opts must be 0 or 1");
+    if (la->sph_degree < 0 || la->sph_degree > 3) return fail(t, PTIR_ERR_INVALID_ARGUMENT, "sph_degree must be in [0,3]");
+    const int ncoef = (la->sph_degree + 1) * (la->sph_degree + 1);
+    if (sc->sh_stride < 3 * ncoef)
+        return fail(t, PTIR_ERR_INVALID_ARGUMENT, "particle_radiance has fewer coefficients than sph_degree needs");
+    if (sc->environment && (sc->env_h <= 0 || sc->env_w <= 0))
+        return fail(t, PTIR_ERR_INVALID_ARGUMENT, "environment with a non-positive size");
+    if (sc->environment && sc->env_h == 6 * sc->env_w)
+        return fail(t, PTIR_ERR_UNSUPPORTED, "cube-map environments ([6N,N,4]) are not supported");
+    if (sc->env_table && (sc->tab_h <= 0 || sc->tab_w <= 0))
+        return fail(t, PTIR_ERR_INVALID_ARGUMENT, "environment table with a non-positive size");
+    return PTIR_OK;
+}
+
+ptir::DeviceScene make_scene(ptir_tracer* t, const ptir_scene* sc, const ptir_launch* la) {
+    ptir::DeviceScene S{};
+    const auto& b = t->bvh;
+    S.grec = b.grec;
+    S.srec = b.srec;
+    S.prec = b.prec;
+    S.nodes = b.nodes;
+    S.bounds = b.bounds;
+    S.sh = sc->particle_radiance;
+    S.env = reinterpret_cast<const float4*>(sc->environment);
+    S.env_w = sc->environment ? sc->env_w : 0;
+    S.env_h = sc->environment ? sc->env_h : 0;
+    S.tab = sc->env_table;
+    S.tab_w = sc->env_table ? sc->tab_w : 0;
+    S.tab_h = sc->env_table ? sc->tab_h : 0;
+    S.ray_to_world = la->ray_to_world;
+    S.ray_ori = la->ray_ori;
+    S.ray_dir = la->ray_dir;
+    S.n = b.n;
+    S.root = b.n > 1 ? 0 : ~0;
+    S.sh_stride = sc->sh_stride;
+    S.sph_degree = la->sph_degree;
+    S.kernel_degree = int(t->opts.kernel_degree);
+    S.min_response = t->opts.min_response;
+    S.min_transmittance = la->min_transmittance;
+    // OptixTracer::trace: maxBounces = renderOpts ? min(maxBounces, 2) : maxBounces (optixTracer.cpp:959)
+    S.max_bounces = la->render_opts ? (la->max_bounces < 2u ? la->max_bounces : 2u) : la->max_bounces;
+    S.render_opts = la->render_opts;
+    S.frame = la->frame;
+    S.B = la->B;
+    S.H = la->H;
+    S.W = la->W;
+    S.enable_normals = t->opts.enable_normals;
+    S.enable_hitcounts = t->opts.enable_hitcounts;
+    S.enable_mis = t->opts.enable_mis;
+    S.enable_metallic = t->opts.enable_metallic;
+    S.has_snormal = sc->shading_normal != nullptr;
+    S.counters = nullptr;
+    S.error_flag = b.error_flag;
+    return S;
+}
+
+ptir_status begin_counters(ptir_tracer* t, ptir::DeviceScene& S, ptir_counters* c, hipStream_t s) {
+    hipError_t e = ptir::fill_zero(t->bvh.error_flag, sizeof(int32_t), s);
+    if (e != hipSuccess) return hip_fail(t, e, "zeroing the error flag");
+    if (!c) return PTIR_OK;
+    e = ptir::fill_zero(t->bvh.counters, ptir::C_COUNT * sizeof(unsigned long long), s);
+    if (e != hipSuccess) return hip_fail(t, e, "zeroing counters");
+    S.counters = t->bvh.counters;
+    return PTIR_OK;
+}
+
+ptir_status end_counters(ptir_tracer* t, ptir_counters* c, hipStream_t s) {
+    if (!c) return PTIR_OK;
+    unsigned long long h[ptir::C_COUNT];
+    int32_t flag = 0;
+    hipError_t e = hipMemcpyAsync(h, t->bvh.counters, sizeof(h), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(&flag, t->bvh.error_flag, sizeof(flag), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(t, e, "reading counters");
+    c->paths = h[ptir::C_PATHS];
+    c->segments = h[ptir::C_SEGMENTS];
+    c->candidates = h[ptir::C_CANDIDATES];
+    c->accepted = h[ptir::C_ACCEPTED];
+    c->accepted_primary = h[ptir::C_ACCEPTED_PRIMARY];
+    c->shadow_candidates = h[ptir::C_SHADOW_CANDIDATES];
+    c->env_lookups = h[ptir::C_ENV_LOOKUPS];
+    c->mat_atomics = h[ptir::C_MAT_ATOMICS];
+    c->env_atomics = h[ptir::C_ENV_ATOMICS];
+    if (flag) return fail(t, PTIR_ERR_BVH_DEPTH, "traversal stack overflow");
+    return PTIR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+ptir_status ptir_create(const ptir_options* options, ptir_tracer** out) {
+    if (!options || !out) return fail(nullptr, PTIR_ERR_INVALID_ARGUMENT, "null options or output handle");
+    *out = nullptr;
+    if (!valid_degree(options->kernel_degree))
+        return fail(nullptr, PTIR_ERR_UNSUPPORTED, "kernel_degree must be one of 0,1,2,3,4,5,8");
+    if (!(options->min_response > 0.f && options->min_response < 1.f))
+        return fail(nullptr, PTIR_ERR_INVALID_ARGUMENT, "min_response must be in (0,1)");
+    if (options->sph_max_degree < 0 || options->sph_max_degree > 3)
+        return fail(nullptr, PTIR_ERR_UNSUPPORTED, "particle_radiance_sph_degree must be in [0,3]");
+    if (options->visualize_lights) return fail(nullptr, PTIR_ERR_UNSUPPORTED, "visualize_lights is not supported");
+    if (options->kbuffer != 8 && options->kbuffer != 16 && options->kbuffer != 32)
+        return fail(nullptr, PTIR_ERR_INVALID_ARGUMENT, "kbuffer must be 8, 16 or 32");
+    ptir_tracer* t = new (std::nothrow) ptir_tracer();
+    if (!t) return fail(nullptr, PTIR_ERR_OUT_OF_MEMORY, "host allocation failed");
+    t->opts = *options;
+    *out = t;
+    return PTIR_OK;
+}
+
+ptir_status ptir_destroy(ptir_tracer* t) {
+    if (!t) return PTIR_OK;
+    ptir::bvh_free(t->bvh);
+    delete t;
+    return PTIR_OK;
+}
+
+ptir_status ptir_build_bvh(ptir_tracer* t, const float* pos, const float* rot, const float* scl, const float* dns,
+                           uint32_t n, uint32_t rebuild, int32_t allow_update, void* stream) {
+    (void)allow_update;
+    if (!t) return fail(nullptr, PTIR_ERR_INVALID_ARGUMENT, "null tracer");
+    if (n > 0 && (!pos || !rot || !scl || !dns)) return fail(t, PTIR_ERR_INVALID_ARGUMENT, "null Gaussian tensor");
+    if (n >= (1u << 30)) return fail(t, PTIR_ERR_INVALID_ARGUMENT, "too many Gaussians");
+    if (!rebuild && (!t->built || t->bvh.n != n))
+        return fail(t, PTIR_ERR_INVALID_ARGUMENT, "refit requested but the hierarchy was built for a different N");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    t->built = false;
+    if (n == 0) {
+        t->bvh.n = 0;
+        t->built = true;
+        return PTIR_OK;
+    }
+    hipError_t e = ptir::bvh_build(t->bvh, kopts(t->opts), pos, rot, scl, dns, n, rebuild != 0, s);
+    if (e != hipSuccess) return hip_fail(t, e, "bvh_build");
+    if (rebuild) {
+        int32_t depth = 0;
+        e = ptir::bvh_depth(t->bvh, s, &depth);
+        if (e != hipSuccess) return hip_fail(t, e, "bvh_depth");
+        if (depth + 1 > ptir::kStackDepth)
+            return fail(t, PTIR_ERR_BVH_DEPTH, "hierarchy depth " + std::to_string(depth) + " exceeds the traversal stack");
+    }
+    t->built = true;
+    return PTIR_OK;
+}
+
+ptir_status ptir_trace(ptir_tracer* t, const ptir_scene* sc, const ptir_launch* la, const ptir_fwd_outputs* out,
+                       ptir_counters* counters, void* stream) {
+    if (!t) return fail(nullptr, PTIR_ERR_INVALID_ARGUMENT, "null tracer");
+    ptir_status st = check_scene(t, sc, la);
+    if (st != PTIR_OK) return st;
+    if (!out || !out->radiance || !out->density || !out->hit_distance || !out->hit_distance2 || !out->distortion ||
+        !out->normal || !out->shading_normal || !out->material || !out->hits_count || !out->pbr || !out->light ||
+        !out->pbr_components || (sc->num_gaussians && !out->visibility))
+        return fail(t, PTIR_ERR_INVALID_ARGUMENT, "null output tensor");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipError_t e = ptir::prep_records(t->bvh, kopts(t->opts), sc->particle_density, sc->particle_material,
+                                      sc->shading_normal, s);
+    if (e != hipSuccess) return hip_fail(t, e, "prep_records");
+    ptir::DeviceScene S = make_scene(t, sc, la);
+    if ((st = begin_counters(t, S, counters, s)) != PTIR_OK) return st;
+    ptir::FwdOut o{out->radiance, out->density, out->hit_distance, out->hit_distance2, out->distortion, out->normal,
+                   out->shading_normal, out->material, out->hits_count, out->pbr, out->light, out->pbr_components,
+                   reinterpret_cast<uint32_t*>(out->visibility)};
+    e = ptir::launch_trace_fwd(S, o, t->opts.kbuffer, s);
+    if (e != hipSuccess) return hip_fail(t, e, "trace kernel launch");
+    return end_counters(t, counters, s);
+}
+
+ptir_status ptir_trace_bwd(ptir_tracer* t, const ptir_scene* sc, const ptir_launch* la, const ptir_bwd_inputs* in,
+                           const ptir_bwd_outputs* out, ptir_counters* counters, void* stream) {
+    if (!t) return fail(nullptr, PTIR_ERR_INVALID_ARGUMENT, "null tracer");
+    ptir_status st = check_scene(t, sc, la);
+    if (st != PTIR_OK) return st;
+    if (!in || !out) return fail(t, PTIR_ERR_INVALID_ARGUMENT, "null backward inputs or outputs");
+    const long long paths = (long long)la->B * la->H * la->W;
+    if (paths > 0 && (!in->pbr || !in->light || !in->pbr_grad))
+        return fail(t, PTIR_ERR_INVALID_ARGUMENT, "null pbr, light or pbr_grad");
+    const size_t N = sc->num_gaussians;
+    if (N && (!out->density_grad || !out->material_grad || !out->radiance_grad || !out->shading_normal_grad))
+        return fail(t, PTIR_ERR_INVALID_ARGUMENT, "null gradient output");
+    if (sc->environment && !out->environment_grad)
+        return fail(t, PTIR_ERR_INVALID_ARGUMENT, "null environment gradient");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipError_t e = hipSuccess;
+    // the reference allocates every gradient with torch::zeros (optixTracer.cpp:1058-1063)
+    if (N) {
+        if (e == hipSuccess) e = ptir::fill_zero(out->density_grad, N * 12 * sizeof(float), s);
+        if (e == hipSuccess) e = ptir::fill_zero(out->radiance_grad, N * size_t(sc->sh_stride) * sizeof(float), s);
+        if (e == hipSuccess) e = ptir::fill_zero(out->shading_normal_grad, N * 3 * sizeof(float), s);
+        if (e == hipSuccess) e = ptir::fill_zero(t->bvh.dmat_leaf, N * 5 * sizeof(float), s);
+    }
+    if (e == hipSuccess && sc->environment)
+        e = ptir::fill_zero(out->environment_grad, size_t(sc->env_h) * sc->env_w * 4 * sizeof(float), s);
+    if (e != hipSuccess) return hip_fail(t, e, "zeroing gradients");
+    e = ptir::prep_records(t->bvh, kopts(t->opts), sc->particle_density, sc->particle_material, sc->shading_normal, s);
+    if (e != hipSuccess) return hip_fail(t, e, "prep_records");
+    ptir::DeviceScene S = make_scene(t, sc, la);
+    if ((st = begin_counters(t, S, counters, s)) != PTIR_OK) return st;
+    ptir::BwdIO io{in->pbr, in->light, in->pbr_grad, in->light_grad, in->material_grad, t->bvh.dmat_leaf,
+                   sc->environment ? out->environment_grad : nullptr};
+    e = ptir::launch_trace_bwd(S, io, t->opts.kbuffer, s);
+    if (e != hipSuccess) return hip_fail(t, e, "trace_bwd kernel launch");
+    if (N) {
+        e = ptir::scatter_material_grad(t->bvh, out->material_grad, s);
+        if (e != hipSuccess) return hip_fail(t, e, "scatter_material_grad");
+    }
+    return end_counters(t, counters, s);
+}
+
+const char* ptir_last_error(const ptir_tracer* t) { return t ? t->err.c_str() : g_create_error.c_str(); }
+
+const char* ptir_version(void) { return "ptir_amd 0.1.0 (gfx950)"; }
+
+}  // extern "C"

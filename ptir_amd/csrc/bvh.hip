@@ -1,0 +1,365 @@
+// LBVH over the Gaussians' instance boxes, built on the GPU.
+//
+// The reference bounds each Gaussian by the unit cube under M = R diag(kscl)
+// (threedgptir_tracer/src/particlePrimitives.cu:543-610, kernelScale :27-51) and lets
+// OptiX build an instance BVH over them (src/optixTracer.cpp:816-904).  Here:
+//   1. k_instances: world AABB of every instance cube and the scene box (order-
+//      preserving integer atomics);
+//   2. k_morton + radix sort: 63-bit Morton codes of the centres;
+//   3. k_karras: binary radix tree (Karras, HPG 2012), duplicate codes split by index;
+//   4. k_refit: bottom-up box fit with one arrival counter per node;
+//   5. k_depth: hierarchy depth, checked against the traversal stack.
+// Node layout (64 B): {lo_l.xyz, link_l} {hi_l.xyz, link_r} {lo_r.xyz, -} {hi_r.xyz, -};
+// link >= 0 is an internal node, link < 0 is leaf slot ~link.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+
+#include "internal.hpp"
+
+namespace ptir {
+
+__device__ __forceinline__ int32_t f2o(float f) {
+    const int32_t b = __float_as_int(f);
+    return b >= 0 ? b : b ^ 0x7FFFFFFF;
+}
+__device__ __forceinline__ float o2f(int32_t o) { return __int_as_float(o >= 0 ? o : o ^ 0x7FFFFFFF); }
+
+// kernelScale of particlePrimitives.cu:27-51
+__device__ __forceinline__ float kernel_scale(float density, const KernelOpts& o) {
+    const float modulation = o.density_clamping ? density : 1.0f;
+    const float minR = fminf(o.min_response / modulation, 0.97f);
+    const float deg = float(o.kernel_degree);
+    if (o.kernel_degree == 0) return ((1.0f - minR) / 3.0f) / -0.329630334487f;
+    const float a = -4.5f / powf(3.0f, deg);
+    return powf(logf(minR) / a, 1.0f / deg);
+}
+
+// standard rotation matrix (column-vector convention) of a (w,x,y,z) quaternion
+__device__ __forceinline__ void quat_to_rot(float4 q, float m[3][3]) {
+    const float w = q.x, x = q.y, y = q.z, z = q.w;
+    m[0][0] = 1.f - 2.f * (y * y + z * z); m[0][1] = 2.f * (x * y - w * z); m[0][2] = 2.f * (x * z + w * y);
+    m[1][0] = 2.f * (x * y + w * z); m[1][1] = 1.f - 2.f * (x * x + z * z); m[1][2] = 2.f * (y * z - w * x);
+    m[2][0] = 2.f * (x * z - w * y); m[2][1] = 2.f * (y * z + w * x); m[2][2] = 1.f - 2.f * (x * x + y * y);
+}
+
+__global__ void k_init_bounds(int32_t* bi) {
+    if (threadIdx.x < 3) bi[threadIdx.x] = f2o(__int_as_float(0x7f800000));
+    else if (threadIdx.x < 6) bi[threadIdx.x] = f2o(-__int_as_float(0x7f800000));
+}
+
+__global__ void __launch_bounds__(256) k_instances(const float* __restrict__ pos, const float* __restrict__ rot,
+                                                   const float* __restrict__ scl, const float* __restrict__ dns,
+                                                   uint32_t n, KernelOpts o, float4* __restrict__ aabb,
+                                                   int32_t* __restrict__ bounds_i) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    float lo[3] = {__int_as_float(0x7f800000), __int_as_float(0x7f800000), __int_as_float(0x7f800000)};
+    float hi[3] = {-lo[0], -lo[1], -lo[2]};
+    if (i < n) {
+        float m[3][3];
+        quat_to_rot(make_float4(rot[4 * i], rot[4 * i + 1], rot[4 * i + 2], rot[4 * i + 3]), m);
+        const float ks = kernel_scale(dns[i], o);
+        const float k[3] = {ks * scl[3 * i], ks * scl[3 * i + 1], ks * scl[3 * i + 2]};
+        float bl[3], bh[3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const float mu = pos[3 * i + r];
+            const float ext = fabsf(m[r][0]) * k[0] + fabsf(m[r][1]) * k[1] + fabsf(m[r][2]) * k[2];
+            // conservative pad: the traversal slab test must never cull a peak on the face
+            const float pad = 2e-6f * (fabsf(mu) + ext) + 1e-7f;
+            bl[r] = mu - ext - pad;
+            bh[r] = mu + ext + pad;
+            lo[r] = bl[r];
+            hi[r] = bh[r];
+        }
+        aabb[2 * i] = make_float4(bl[0], bl[1], bl[2], 0.f);
+        aabb[2 * i + 1] = make_float4(bh[0], bh[1], bh[2], 0.f);
+    }
+    // wave reduction then one atomic per wave
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        for (int off = 32; off > 0; off >>= 1) {
+            lo[r] = fminf(lo[r], __shfl_xor(lo[r], off));
+            hi[r] = fmaxf(hi[r], __shfl_xor(hi[r], off));
+        }
+    }
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            atomicMin(bounds_i + r, f2o(lo[r]));
+            atomicMax(bounds_i + 3 + r, f2o(hi[r]));
+        }
+    }
+}
+
+__device__ __forceinline__ unsigned long long spread21(unsigned long long v) {
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffull;
+    v = (v | v << 16) & 0x1f0000ff0000ffull;
+    v = (v | v << 8) & 0x100f00f00f00f00full;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ull;
+    v = (v | v << 2) & 0x1249249249249249ull;
+    return v;
+}
+
+__global__ void __launch_bounds__(256) k_morton(const float4* __restrict__ aabb, uint32_t n,
+                                                const int32_t* __restrict__ bounds_i, float* __restrict__ bounds,
+                                                unsigned long long* __restrict__ keys, uint32_t* __restrict__ idx) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const float lo[3] = {o2f(bounds_i[0]), o2f(bounds_i[1]), o2f(bounds_i[2])};
+    const float hi[3] = {o2f(bounds_i[3]), o2f(bounds_i[4]), o2f(bounds_i[5])};
+    if (i == 0) {
+        for (int r = 0; r < 3; ++r) {
+            bounds[r] = lo[r];
+            bounds[3 + r] = hi[r];
+        }
+    }
+    if (i >= n) return;
+    const float4 a = aabb[2 * i], b = aabb[2 * i + 1];
+    const float c[3] = {0.5f * (a.x + b.x), 0.5f * (a.y + b.y), 0.5f * (a.z + b.z)};
+    unsigned long long q[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        const float ext = hi[r] - lo[r];
+        float u = ext > 0.f ? (c[r] - lo[r]) / ext : 0.5f;
+        u = fminf(fmaxf(u, 0.f), 1.f);
+        q[r] = (unsigned long long)fminf(u * 2097152.f, 2097151.f);
+    }
+    keys[i] = (spread21(q[0]) << 2) | (spread21(q[1]) << 1) | spread21(q[2]);
+    idx[i] = i;
+}
+
+__device__ __forceinline__ int delta(const unsigned long long* __restrict__ k, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    const unsigned long long a = k[i], b = k[j];
+    if (a == b) return 64 + __clz(uint32_t(i ^ j));
+    return __clzll(a ^ b);
+}
+
+__global__ void __launch_bounds__(256) k_karras(const unsigned long long* __restrict__ k, int n,
+                                                float4* __restrict__ nodes, int32_t* __restrict__ parent,
+                                                uint32_t* __restrict__ flags) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    const int d = (delta(k, n, i, i + 1) - delta(k, n, i, i - 1)) >= 0 ? 1 : -1;
+    const int dmin = delta(k, n, i, i - d);
+    int lmax = 2;
+    while (delta(k, n, i, i + lmax * d) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (delta(k, n, i, i + (l + t) * d) > dmin) l += t;
+    const int j = i + l * d;
+    const int dnode = delta(k, n, i, j);
+    int s = 0;
+    int t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (delta(k, n, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    const int gamma = i + s * d + (d < 0 ? -1 : 0);
+    const int lo = i < j ? i : j, hi = i < j ? j : i;
+    const int left = (lo == gamma) ? ~gamma : gamma;
+    const int right = (hi == gamma + 1) ? ~(gamma + 1) : gamma + 1;
+    float4* nd = nodes + 4 * i;
+    nd[0].w = __int_as_float(left);
+    nd[1].w = __int_as_float(right);
+    parent[left >= 0 ? left : (n - 1) + ~left] = i;
+    parent[right >= 0 ? right : (n - 1) + ~right] = i;
+    flags[i] = 0u;
+    if (i == 0) parent[0] = -1;
+}
+
+__global__ void __launch_bounds__(256) k_refit(const float4* __restrict__ aabb, const uint32_t* __restrict__ leaf_id,
+                                               int n, float4* nodes, const int32_t* __restrict__ parent,
+                                               uint32_t* flags) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t g = leaf_id[j];
+    float4 lo = aabb[2 * g], hi = aabb[2 * g + 1];
+    int child = ~j;
+    int p = parent[(n - 1) + j];
+    while (p >= 0) {
+        float4* nd = nodes + 4 * p;
+        const bool is_left = __float_as_int(nd[0].w) == child;
+        float4* dlo = is_left ? &nd[0] : &nd[2];
+        float4* dhi = is_left ? &nd[1] : &nd[3];
+        dlo->x = lo.x; dlo->y = lo.y; dlo->z = lo.z;
+        dhi->x = hi.x; dhi->y = hi.y; dhi->z = hi.z;
+        __threadfence();
+        if (atomicAdd(&flags[p], 1u) == 0u) return;  // first arrival: the sibling finishes
+        __threadfence();
+        const float4 a0 = nd[0], a1 = nd[1];
+        const float4 b0 = nd[2], b1 = nd[3];
+        lo = make_float4(fminf(a0.x, b0.x), fminf(a0.y, b0.y), fminf(a0.z, b0.z), 0.f);
+        hi = make_float4(fmaxf(a1.x, b1.x), fmaxf(a1.y, b1.y), fmaxf(a1.z, b1.z), 0.f);
+        child = p;
+        p = parent[p];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_depth(const int32_t* __restrict__ parent, int n, int32_t* depth) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    int d = 0;
+    for (int p = parent[(n - 1) + j]; p >= 0; p = parent[p]) ++d;
+    atomicMax(depth, d);
+}
+
+__global__ void k_clear_flags(uint32_t* flags, int m) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) flags[i] = 0u;
+}
+
+// Per-trace records in leaf order from the trace's own particle tensors
+// (processHit reads particleDensity/particleMaterial/shadingNormal, gaussianParticles.cuh:343-440).
+__global__ void __launch_bounds__(256) k_records(const float* __restrict__ density, const float* __restrict__ material,
+                                                 const float* __restrict__ snormal,
+                                                 const uint32_t* __restrict__ leaf_id, uint32_t n, KernelOpts o,
+                                                 float4* __restrict__ grec, float4* __restrict__ srec,
+                                                 float4* __restrict__ prec) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t i = leaf_id[j];
+    const float* p = density + 12 * size_t(i);
+    float m[3][3];
+    quat_to_rot(make_float4(p[4], p[5], p[6], p[7]), m);
+    const float s[3] = {p[8], p[9], p[10]};
+    const float is[3] = {1.0f / s[0], 1.0f / s[1], 1.0f / s[2]};
+    // rows of A = diag(1/s) R^T: row r is column r of R scaled by 1/s_r
+    grec[4 * j + 0] = make_float4(m[0][0] * is[0], m[1][0] * is[0], m[2][0] * is[0], p[0]);
+    grec[4 * j + 1] = make_float4(m[0][1] * is[1], m[1][1] * is[1], m[2][1] * is[1], p[1]);
+    grec[4 * j + 2] = make_float4(m[0][2] * is[2], m[1][2] * is[2], m[2][2] * is[2], p[2]);
+    grec[4 * j + 3] = make_float4(p[3], kernel_scale(p[3], o), __int_as_float(int(i)), 0.f);
+    const float* mt = material + 5 * size_t(i);
+    const float sn[3] = {snormal ? snormal[3 * i] : 0.f, snormal ? snormal[3 * i + 1] : 0.f,
+                         snormal ? snormal[3 * i + 2] : 0.f};
+    srec[2 * j + 0] = make_float4(mt[0], mt[1], mt[2], mt[3]);
+    srec[2 * j + 1] = make_float4(mt[4], sn[0], sn[1], sn[2]);
+    prec[j] = make_float4(m[0][0] * s[0] + m[0][1] * s[1] + m[0][2] * s[2],
+                          m[1][0] * s[0] + m[1][1] * s[1] + m[1][2] * s[2],
+                          m[2][0] * s[0] + m[2][1] * s[1] + m[2][2] * s[2], snormal ? 1.f : 0.f);
+}
+
+__global__ void __launch_bounds__(256) k_scatter_mat(const float* __restrict__ leaf, const uint32_t* __restrict__ leaf_id,
+                                                     uint32_t n, float* __restrict__ out) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t i = leaf_id[j];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) out[5 * size_t(i) + c] = leaf[5 * size_t(j) + c];
+}
+
+__global__ void __launch_bounds__(256) k_fill_zero(uint32_t* __restrict__ p, size_t words) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < words; i += size_t(gridDim.x) * blockDim.x)
+        p[i] = 0u;
+}
+
+hipError_t fill_zero(void* p, size_t bytes, hipStream_t s) {
+    if (!bytes) return hipSuccess;
+    const size_t words = bytes / 4;  // every buffer zeroed here is a float/int array
+    const unsigned grid = unsigned(std::min<size_t>((words + 255) / 256, 8192));
+    k_fill_zero<<<grid, 256, 0, s>>>(static_cast<uint32_t*>(p), words);
+    return hipGetLastError();
+}
+
+template <class T>
+static hipError_t grow(T*& p, size_t count) {
+    if (p) {
+        hipError_t e = hipFree(p);
+        p = nullptr;
+        if (e != hipSuccess) return e;
+    }
+    return hipMalloc((void**)&p, count * sizeof(T) + 256);
+}
+
+hipError_t bvh_reserve(BvhBuffers& b, uint32_t n) {
+    if (n <= b.cap && b.aabb) return hipSuccess;
+    hipError_t e;
+    const size_t N = n, M = n > 1 ? n - 1 : 1;
+#define G(ptr, cnt) if ((e = grow(ptr, cnt)) != hipSuccess) return e;
+    G(b.aabb, 2 * N);
+    G(b.keys, N);
+    G(b.keys_sorted, N);
+    G(b.idx, N);
+    G(b.leaf_id, N);
+    G(b.nodes, 4 * M);
+    G(b.parent, 2 * N);
+    G(b.flags, M);
+    G(b.grec, 4 * N);
+    G(b.srec, 2 * N);
+    G(b.prec, N);
+    G(b.dmat_leaf, 5 * N);
+#undef G
+    if (!b.bounds_i) {
+        if ((e = hipMalloc((void**)&b.bounds_i, 6 * sizeof(int32_t))) != hipSuccess) return e;
+        if ((e = hipMalloc((void**)&b.bounds, 6 * sizeof(float))) != hipSuccess) return e;
+        if ((e = hipMalloc((void**)&b.depth, sizeof(int32_t))) != hipSuccess) return e;
+        if ((e = hipMalloc((void**)&b.counters, C_COUNT * sizeof(unsigned long long))) != hipSuccess) return e;
+        if ((e = hipMalloc((void**)&b.error_flag, sizeof(int32_t))) != hipSuccess) return e;
+    }
+    size_t tmp = 0;
+    e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, b.keys, b.keys_sorted, b.idx, b.leaf_id, int(n), 0, 63);
+    if (e != hipSuccess) return e;
+    if (tmp > b.sort_tmp_bytes) {
+        if (b.sort_tmp) hipFree(b.sort_tmp);
+        b.sort_tmp = nullptr;
+        if ((e = hipMalloc(&b.sort_tmp, tmp)) != hipSuccess) return e;
+        b.sort_tmp_bytes = tmp;
+    }
+    b.cap = n;
+    return hipSuccess;
+}
+
+void bvh_free(BvhBuffers& b) {
+    void* ptrs[] = {b.aabb, b.keys, b.keys_sorted, b.idx, b.leaf_id, b.nodes, b.parent, b.flags, b.bounds_i,
+                    b.bounds, b.depth, b.sort_tmp, b.grec, b.srec, b.prec, b.dmat_leaf, b.counters, b.error_flag};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    b = BvhBuffers();
+}
+
+static inline unsigned blocks(uint32_t n, unsigned bs) { return (n + bs - 1) / bs; }
+
+hipError_t bvh_build(BvhBuffers& b, const KernelOpts& o, const float* pos, const float* rot, const float* scl,
+                     const float* dns, uint32_t n, bool rebuild, hipStream_t s) {
+    hipError_t e = bvh_reserve(b, n);
+    if (e != hipSuccess) return e;
+    b.n = n;
+    k_init_bounds<<<1, 64, 0, s>>>(b.bounds_i);
+    k_instances<<<blocks(n, 256), 256, 0, s>>>(pos, rot, scl, dns, n, o, b.aabb, b.bounds_i);
+    k_morton<<<blocks(n, 256), 256, 0, s>>>(b.aabb, n, b.bounds_i, b.bounds, b.keys, b.idx);
+    if (rebuild) {
+        e = hipcub::DeviceRadixSort::SortPairs(b.sort_tmp, b.sort_tmp_bytes, b.keys, b.keys_sorted, b.idx, b.leaf_id,
+                                               int(n), 0, 63, s);
+        if (e != hipSuccess) return e;
+        if (n > 1) k_karras<<<blocks(n - 1, 256), 256, 0, s>>>(b.keys_sorted, int(n), b.nodes, b.parent, b.flags);
+    } else if (n > 1) {
+        k_clear_flags<<<blocks(n - 1, 256), 256, 0, s>>>(b.flags, int(n - 1));
+    }
+    if (n > 1) k_refit<<<blocks(n, 256), 256, 0, s>>>(b.aabb, b.leaf_id, int(n), b.nodes, b.parent, b.flags);
+    return hipGetLastError();
+}
+
+hipError_t bvh_depth(BvhBuffers& b, hipStream_t s, int32_t* host_depth) {
+    hipError_t e = fill_zero(b.depth, sizeof(int32_t), s);
+    if (e != hipSuccess) return e;
+    if (b.n > 1) k_depth<<<blocks(b.n, 256), 256, 0, s>>>(b.parent, int(b.n), b.depth);
+    e = hipMemcpyAsync(host_depth, b.depth, sizeof(int32_t), hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+}
+
+hipError_t prep_records(BvhBuffers& b, const KernelOpts& o, const float* density, const float* material,
+                        const float* snormal, hipStream_t s) {
+    if (b.n == 0) return hipSuccess;
+    k_records<<<blocks(b.n, 256), 256, 0, s>>>(density, material, snormal, b.leaf_id, b.n, o, b.grec, b.srec, b.prec);
+    return hipGetLastError();
+}
+
+hipError_t scatter_material_grad(const BvhBuffers& b, float* dmat, hipStream_t s) {
+    if (b.n == 0) return hipSuccess;
+    k_scatter_mat<<<blocks(b.n, 256), 256, 0, s>>>(b.dmat_leaf, b.leaf_id, b.n, dmat);
+    return hipGetLastError();
+}
+
+}  // namespace ptir

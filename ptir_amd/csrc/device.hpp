@@ -1,0 +1,154 @@
+// Device-side types and helpers of the gfx950 path tracer.
+//
+// Scene data lives in HBM in "leaf order" (the order of the LBVH leaves, i.e. Morton
+// order of the Gaussian centres) so that rays that traverse neighbouring leaves touch
+// neighbouring records:
+//   GaussRec  64 B  rows of A = diag(1/s) R^T with mu in .w, then (sigma, kscale, id, -)
+//   ShadeRec  32 B  (albedo.xyz, roughness) (metallic, shading normal.xyz)
+//   PrimRec   16 B  (R s).xyz   used only by the primary-ray normal output
+// A per-lane gather of one record is one 64 B segment, which is why the records are
+// array-of-structures: lanes of a wave follow different BVH leaves, so a
+// structure-of-arrays layout would touch 16 separate cache lines per candidate.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ptir {
+
+// ---------------------------------------------------------------- float3 math
+struct V3 {
+    float x, y, z;
+};
+__device__ __forceinline__ V3 v3(float a, float b, float c) { return V3{a, b, c}; }
+__device__ __forceinline__ V3 v3s(float a) { return V3{a, a, a}; }
+__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 operator-(V3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ V3 operator*(V3 a, V3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ V3 operator*(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ V3 operator*(float s, V3 a) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ V3 operator/(V3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ V3& operator+=(V3& a, V3 b) { a = a + b; return a; }
+__device__ __forceinline__ V3& operator-=(V3& a, V3 b) { a = a - b; return a; }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ bool is_zero(V3 a) { return a.x == 0.f && a.y == 0.f && a.z == 0.f; }
+__device__ __forceinline__ float vmax(V3 a) { return fmaxf(a.x, fmaxf(a.y, a.z)); }
+__device__ __forceinline__ V3 xyz(float4 a) { return {a.x, a.y, a.z}; }
+__device__ __forceinline__ V3 safe_normalize(V3 v) {
+    const float l = dot(v, v);
+    return l > 0.f ? v * rsqrtf(l) : v;
+}
+// Duff et al., "Building an Orthonormal Basis, Revisited" (JCGT 2017).
+__device__ __forceinline__ void onb(V3 n, V3& b1, V3& b2) {
+    const float sign = copysignf(1.0f, n.z);
+    const float a = -1.0f / (sign + n.z);
+    const float b = n.x * n.y * a;
+    b1 = v3(1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x);
+    b2 = v3(b, sign + n.y * n.y * a, -n.y);
+}
+
+// ---------------------------------------------------------------- constants
+constexpr float kPiF = 3.14159265358979323846f;
+constexpr float kSelfOccOffset = 2e-2f;
+constexpr float kMaxSelfOcc = 1e-1f - 2e-2f;
+constexpr float kSelfOccDot = 0.1f;
+constexpr float kBrdfEps = 1e-6f;
+constexpr float kMinRough = 0.05f;
+constexpr float kRoughGradEps = 1e-3f;
+constexpr float kEnvDirEps = 1e-3f;
+constexpr float kInfDist = 1e20f;
+constexpr float kEpsT = 1e-9f;
+constexpr float kAlphaMin = 1.0f / 255.0f;
+constexpr uint32_t kInvalid = 0xFFFFFFFFu;
+
+// ---------------------------------------------------------------- PCG32 (O'Neill, XSH-RR)
+struct Rng {
+    uint64_t state, inc;
+    __device__ __forceinline__ static uint64_t launch_seed(uint32_t x, uint32_t y, uint32_t z, uint32_t f) {
+        uint64_t s = 1469598103934665603ull;
+        s ^= uint64_t(x) + 1099511628211ull * s;
+        s ^= uint64_t(y) + 1099511628211ull * s;
+        s ^= uint64_t(z) + 1099511628211ull * s;
+        s ^= uint64_t(f) + 1099511628211ull * s;
+        return s ? s : 1ull;
+    }
+    __device__ __forceinline__ void seed(uint64_t s) {
+        state = 0;
+        inc = 1ull;
+        next_u32();
+        state += s;
+        next_u32();
+    }
+    __device__ __forceinline__ uint32_t next_u32() {
+        const uint64_t old = state;
+        state = old * 6364136223846793005ull + inc;
+        const uint32_t xs = uint32_t(((old >> 18u) ^ old) >> 27u);
+        const uint32_t rot = uint32_t(old >> 59u);
+        return (xs >> rot) | (xs << ((0u - rot) & 31u));
+    }
+    __device__ __forceinline__ float next_1d() { return float(next_u32() & 0x00FFFFFFu) * (1.0f / 16777216.0f); }
+};
+
+// ---------------------------------------------------------------- scene description
+struct DeviceScene {
+    const float4* grec;       // [N*4]   GaussRec, leaf order
+    const float4* srec;       // [N*2]   ShadeRec, leaf order
+    const float4* prec;       // [N]     PrimRec, leaf order
+    const float4* nodes;      // [(N-1)*4] internal nodes: child boxes + child links
+    const float* bounds;      // [6] scene AABB lo.xyz hi.xyz (device)
+    const float* sh;          // [N, sh_stride] original order
+    const float4* env;        // [env_h, env_w]
+    const float* tab;         // [3, tab_h, tab_w]
+    const float* ray_to_world;  // 12 floats
+    const float* ray_ori;     // [B,H,W,3]
+    const float* ray_dir;
+    uint32_t n;
+    int32_t root;             // child link of the root (>=0 internal, <0 leaf ~slot), N>0
+    int32_t sh_stride, sph_degree;
+    int32_t env_w, env_h, tab_w, tab_h;
+    int32_t kernel_degree;
+    float min_response, min_transmittance;
+    uint32_t max_bounces, render_opts, frame;
+    int32_t B, H, W;
+    int32_t enable_normals, enable_hitcounts, enable_mis, enable_metallic, has_snormal;
+    int32_t tiles_x, tiles_y, num_blocks;  // launch tiling
+    unsigned long long* counters;          // 9 counters or null
+    int32_t* error_flag;
+};
+
+struct FwdOut {
+    float *rad, *dns, *hit, *hit2, *dist, *nrm, *snrm, *mat, *hits, *pbr, *light, *comp;
+    uint32_t* vis;  // [N] original order
+};
+
+struct BwdIO {
+    const float *pbr, *light, *pbr_grad, *light_grad, *mat_grad;
+    float* dmat_leaf;  // [N,5] leaf order (scattered to the original order afterwards)
+    float* denv;       // [env_h, env_w, 4]
+};
+
+enum CounterId { C_PATHS = 0, C_SEGMENTS, C_CANDIDATES, C_ACCEPTED, C_ACCEPTED_PRIMARY, C_SHADOW_CANDIDATES,
+                 C_ENV_LOOKUPS, C_MAT_ATOMICS, C_ENV_ATOMICS, C_COUNT };
+
+__device__ __forceinline__ void count(const DeviceScene& S, int id, unsigned long long v = 1) {
+    if (S.counters) atomicAdd(S.counters + id, v);
+}
+
+// generalized Gaussian response exp(-4.5 (r/3)^(2n)) of squared distance d2 = r^2
+__device__ __forceinline__ float response(int degree, float d2) {
+    switch (degree) {
+    case 8: { const float q = d2 * d2; return expf(-0.000685871056241f * q * q); }
+    case 5: return expf(-0.0185185185185f * d2 * d2 * sqrtf(d2));
+    case 4: return expf(-0.0555555555556f * d2 * d2);
+    case 3: return expf(-0.166666666667f * d2 * sqrtf(d2));
+    case 1: return expf(-1.5f * sqrtf(d2));
+    case 0: return fmaxf(1.f - 0.329630334487f * sqrtf(d2), 0.f);
+    default: return expf(-0.5f * d2);
+    }
+}
+
+}  // namespace ptir

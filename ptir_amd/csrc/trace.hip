@@ -1,0 +1,1083 @@
+// Forward and path-replay backward kernels of the differentiable Gaussian path tracer.
+//
+// One lane owns one path sample (launch index x, y, z of the reference,
+// src/kernels/cuda/referenceOptix.cu:27-71 and referenceBwdOptix.cu:27-74).  A 128-lane
+// workgroup covers a 16x8 pixel tile (each wave an 8x8 tile) so that the primary rays
+// of a wave traverse the same part of the hierarchy.  Workgroup ids are remapped so
+// that neighbouring tiles land on the same XCD (and its L2).
+//
+// Hit collection replaces OptiX's any-hit k-buffer (referenceOptix.cu:95-133): a
+// stack-based LBVH walk keeps the K closest candidates with t* in (tmin, tmax) sorted
+// by (t*, leaf slot) in registers and prunes boxes that start beyond the K-th distance.
+// The traversal stack lives in LDS; once the walk ends the same LDS words hold the
+// sorted K-buffer, which the compositing loop then consumes in order.
+#include "internal.hpp"
+
+namespace ptir {
+
+struct Ray {
+    V3 o, d;
+};
+
+// ---------------------------------------------------------------- per-lane LDS view
+template <int K>
+struct LdsLayout {
+    static constexpr int kWords = (2 * K > kStackDepth) ? 2 * K : kStackDepth;
+};
+
+struct Lane {
+    uint32_t* base;  // &lds[threadIdx.x], stride kTraceBlock
+    __device__ __forceinline__ uint32_t& at(int i) const { return base[i * kTraceBlock]; }
+};
+
+// ---------------------------------------------------------------- candidates and traversal
+// Candidate rule (the deterministic restatement of the reference's IS program,
+// gaussianParticles.cuh:484-500): t* = -gro.grdu/|grdu|^2 in (tmin, tmax), the peak
+// point lies in the instance cube |q|_inf <= kscale, and |q|^2 < 9 |grdu|^2.
+__device__ __forceinline__ bool cand_test(const DeviceScene& S, int slot, const Ray& r, float tmin, float tmax,
+                                          float& t) {
+    const float4* g = S.grec + 4 * slot;
+    const float4 a0 = g[0], a1 = g[1], a2 = g[2], p = g[3];
+    const V3 dp = r.o - v3(a0.w, a1.w, a2.w);
+    const V3 gro = v3(dot(xyz(a0), dp), dot(xyz(a1), dp), dot(xyz(a2), dp));
+    const V3 grdu = v3(dot(xyz(a0), r.d), dot(xyz(a1), r.d), dot(xyz(a2), r.d));
+    const float dd = dot(grdu, grdu);
+    t = -dot(gro, grdu) / dd;
+    if (!(t > tmin && t < tmax)) return false;
+    const V3 q = gro + grdu * t;
+    if (fmaxf(fabsf(q.x), fmaxf(fabsf(q.y), fabsf(q.z))) > p.y) return false;
+    return dot(q, q) < 9.0f * dd;
+}
+
+__device__ __forceinline__ bool slab(float4 lo, float4 hi, const Ray& r, V3 inv, float tmin, float tmax, float& tn) {
+    const float ax = (lo.x - r.o.x) * inv.x, bx = (hi.x - r.o.x) * inv.x;
+    const float ay = (lo.y - r.o.y) * inv.y, by = (hi.y - r.o.y) * inv.y;
+    const float az = (lo.z - r.o.z) * inv.z, bz = (hi.z - r.o.z) * inv.z;
+    tn = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fminf(az, bz));
+    const float tf = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
+    return tn <= tf && tf >= tmin && tn <= tmax;
+}
+
+// Collect the K closest candidates in (tmin, tmax).  Returns the count; the sorted
+// entries are left in LDS: t at words [0,K), leaf slot at words [K,2K).
+template <int K>
+__device__ int collect(const DeviceScene& S, const Ray& r, float tmin, float tmax, const Lane& L) {
+    float bt[K];
+    uint32_t bs[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        bt[j] = __int_as_float(0x7f800000);
+        bs[j] = kInvalid;
+    }
+    unsigned long long ncand = 0;
+    auto insert = [&](float t, uint32_t s) {
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const bool lt = (t < bt[j]) || (t == bt[j] && s < bs[j]);
+            const float tt = lt ? bt[j] : t;
+            const uint32_t ss = lt ? bs[j] : s;
+            bt[j] = lt ? t : bt[j];
+            bs[j] = lt ? s : bs[j];
+            t = tt;
+            s = ss;
+        }
+    };
+    auto leaf = [&](int slot) {
+        float t;
+        if (cand_test(S, slot, r, tmin, tmax, t)) insert(t, uint32_t(slot));
+    };
+    const V3 inv = v3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
+    if (S.n > 0) {
+        int link = S.root;
+        if (link < 0) {
+            leaf(~link);
+        } else {
+            int sp = 0;
+            for (;;) {
+                const float4* nd = S.nodes + 4 * link;
+                const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2], n3 = nd[3];
+                const float cut = fminf(tmax, bt[K - 1]);
+                float tl, tr;
+                bool hl = slab(n0, n1, r, inv, tmin, cut, tl);
+                bool hr = slab(n2, n3, r, inv, tmin, cut, tr);
+                const int cl = __float_as_int(n0.w), cr = __float_as_int(n1.w);
+                if (hl && cl < 0) { leaf(~cl); hl = false; }
+                if (hr && cr < 0) { leaf(~cr); hr = false; }
+                if (hl && hr) {
+                    const bool lf = tl <= tr;
+                    if (sp >= kStackDepth) {  // cannot happen after the build-time depth check
+                        atomicOr(S.error_flag, 1);
+                        break;
+                    }
+                    L.at(sp++) = uint32_t(lf ? cr : cl);
+                    link = lf ? cl : cr;
+                } else if (hl) {
+                    link = cl;
+                } else if (hr) {
+                    link = cr;
+                } else {
+                    if (sp == 0) break;
+                    link = int(L.at(--sp));
+                }
+            }
+        }
+    }
+    (void)ncand;
+    int cnt = 0;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        L.at(j) = __float_as_uint(bt[j]);
+        L.at(K + j) = bs[j];
+        cnt += bs[j] != kInvalid ? 1 : 0;
+    }
+    return cnt;
+}
+
+// pathTracer.cuh:46-52 ray vs scene box
+__device__ __forceinline__ void scene_interval(const DeviceScene& S, const Ray& r, float& t0, float& t1) {
+    const float* b = S.bounds;
+    const float ax = (b[0] - r.o.x) / r.d.x, bx = (b[3] - r.o.x) / r.d.x;
+    const float ay = (b[1] - r.o.y) / r.d.y, by = (b[4] - r.o.y) / r.d.y;
+    const float az = (b[2] - r.o.z) / r.d.z, bz = (b[5] - r.o.z) / r.d.z;
+    t0 = fmaxf(0.f, fmaxf(fminf(ax, bx), fmaxf(fminf(ay, by), fminf(az, bz))));
+    t1 = fminf(fmaxf(ax, bx), fminf(fmaxf(ay, by), fmaxf(az, bz)));
+}
+
+// ---------------------------------------------------------------- path state
+struct Seg {  // payLoad.h:38-100 rayPayload
+    Ray ray;
+    bool ivalid;
+    V3 pos, sn, alb;
+    float rough, metal;
+    V3 galb;
+    float grough, gmetal;
+    V3 radiance, light, contribution, normal;
+    float T, hitDist, hitDist2, lastHit, distortion, hits;
+    float scatterPdf, lightPdf;
+    V3 dirGrad;
+    bool hit, valid;
+    __device__ void reset(const Ray& r, float last) {
+        ray = r;
+        ivalid = false;
+        pos = sn = alb = galb = v3s(0.f);
+        rough = metal = grough = gmetal = 0.f;
+        radiance = light = contribution = normal = dirGrad = v3s(0.f);
+        T = 1.f;
+        hitDist = hitDist2 = distortion = hits = 0.f;
+        lastHit = last;
+        scatterPdf = lightPdf = 0.f;
+        hit = valid = false;
+    }
+};
+
+struct Path {  // payLoad.h:142-168
+    unsigned active, numBounces;
+    Seg cur;
+    // emitter payload: only these fields are read
+    Ray eray;
+    float elightPdf, escatterPdf;
+    V3 econtrib;
+    V3 L, dL, direct, indirect, Lnb, dLnb, thr;
+    // PendingRayDirectionGrad
+    bool pvalid;
+    Ray pray;
+    float popacity, pmaxHit;
+    V3 pdir;
+    unsigned pbounces;
+};
+
+__device__ __forceinline__ void emit_clear(Path& p) {
+    p.eray = Ray{v3s(0.f), v3s(0.f)};
+    p.elightPdf = p.escatterPdf = 0.f;
+    p.econtrib = v3s(0.f);
+}
+
+// ---------------------------------------------------------------- environment (environment.cuh)
+__device__ __forceinline__ bool has_env(const DeviceScene& S) { return S.env && S.env_w > 0 && S.env_h > 0; }
+__device__ __forceinline__ bool has_env_light(const DeviceScene& S) {
+    return has_env(S) && S.tab && S.tab_w > 0 && S.tab_h > 0;
+}
+
+// rotation with offset (0,0): rotZ = pi/2, rotX = 0 (environment.cuh:123-158)
+__device__ __forceinline__ V3 env_rotate(V3 d) {
+    float sz, cz;
+    sincosf(0.5f * kPiF, &sz, &cz);
+    return v3(d.x * cz - d.y * sz, d.x * sz + d.y * cz, d.z);
+}
+__device__ __forceinline__ V3 env_inverse_rotate(V3 e) {
+    float sz, cz;
+    sincosf(0.5f * kPiF, &sz, &cz);
+    return v3(e.x * cz + e.y * sz, -e.x * sz + e.y * cz, e.z);
+}
+
+struct Foot {
+    int x0, x1, y0, y1;
+    float w00, w10, w01, w11;
+};
+
+__device__ __forceinline__ Foot footprint(const DeviceScene& S, float u, float v) {
+    u = u - floorf(u);
+    v = fminf(fmaxf(v, 0.f), 1.f);
+    const float x = u * float(S.env_w) - 0.5f, y = v * float(S.env_h) - 0.5f;
+    const int x0 = int(floorf(x)), y0 = int(floorf(y));
+    const float ax = x - float(x0), ay = y - float(y0);
+    auto wrap = [](int v, int s) { int w = v % s; return w < 0 ? w + s : w; };
+    Foot f;
+    f.x0 = wrap(x0, S.env_w);
+    f.x1 = wrap(x0 + 1, S.env_w);
+    f.y0 = max(0, min(y0, S.env_h - 1));
+    f.y1 = max(0, min(y0 + 1, S.env_h - 1));
+    f.w00 = (1.f - ax) * (1.f - ay);
+    f.w10 = ax * (1.f - ay);
+    f.w01 = (1.f - ax) * ay;
+    f.w11 = ax * ay;
+    return f;
+}
+
+__device__ __forceinline__ V3 bilinear(const DeviceScene& S, const Foot& f) {
+    const float4 a = S.env[f.y0 * S.env_w + f.x0], b = S.env[f.y0 * S.env_w + f.x1];
+    const float4 c = S.env[f.y1 * S.env_w + f.x0], d = S.env[f.y1 * S.env_w + f.x1];
+    return xyz(a) * f.w00 + xyz(b) * f.w10 + xyz(c) * f.w01 + xyz(d) * f.w11;
+}
+
+__device__ __forceinline__ void equirect_uv(V3 dir, float& u, float& v) {
+    const float theta = atan2f(dir.x, dir.y);
+    const float zcl = fminf(1.f, fmaxf(-1.f, -dir.z));
+    const float phi = asinf(zcl);
+    u = (theta + kPiF) * (0.5f * (1.0f / kPiF));
+    v = 0.5f + phi * (1.0f / kPiF);
+}
+
+__device__ __forceinline__ V3 env_rotated(const DeviceScene& S, V3 dir) {
+    float u, v;
+    equirect_uv(dir, u, v);
+    count(S, C_ENV_LOOKUPS);
+    return bilinear(S, footprint(S, u, v));
+}
+
+__device__ __forceinline__ V3 background(const DeviceScene& S, V3 rd) {
+    if (!has_env(S)) return v3s(0.f);
+    return env_rotated(S, env_rotate(rd));
+}
+
+// getBackgroundColorBwd + getBackgroundRayDirectionGradFast (environment.cuh:421-473)
+__device__ V3 background_bwd(const DeviceScene& S, V3 rd, V3 cg, V3& dirGrad, float* denv) {
+    if (!has_env(S)) return v3s(0.f);
+    const V3 dir = env_rotate(rd);
+    float u, v;
+    equirect_uv(dir, u, v);
+    const Foot f = footprint(S, u, v);
+    count(S, C_ENV_LOOKUPS);
+    count(S, C_ENV_ATOMICS);
+    const V3 bg = bilinear(S, f);
+    auto add = [&](int x, int y, float w) {
+        float* p = denv + 4 * (size_t(y) * S.env_w + x);
+        atomicAdd(p + 0, w * cg.x);
+        atomicAdd(p + 1, w * cg.y);
+        atomicAdd(p + 2, w * cg.z);
+    };
+    add(f.x0, f.y0, f.w00);
+    add(f.x1, f.y0, f.w10);
+    add(f.x0, f.y1, f.w01);
+    add(f.x1, f.y1, f.w11);
+    const V3 un = safe_normalize(rd);
+    V3 t, b;
+    onb(un, t, b);
+    const float scl = rsqrtf(1.0f + kEnvDirEps * kEnvDirEps);
+    const V3 tb = env_rotated(S, env_rotate((un + kEnvDirEps * t) * scl));
+    const V3 bb = env_rotated(S, env_rotate((un + kEnvDirEps * b) * scl));
+    const float inv = 1.0f / kEnvDirEps;
+    const float tg = dot((tb - bg) * inv, cg);
+    const float bgd = dot((bb - bg) * inv, cg);
+    dirGrad += t * tg + b * bgd;
+    return bg;
+}
+
+// sampleEnvironmentAliasDirection (environment.cuh:227-255); draws u1..u4 in order
+__device__ V3 sample_env(const DeviceScene& S, Rng& rng, float& pdf) {
+    const int nc = S.tab_h * S.tab_w;
+    const float* prob = S.tab;
+    const float* alias = S.tab + nc;
+    const float* pdfs = S.tab + 2 * nc;
+    const int init = min(int(rng.next_1d() * float(nc)), nc - 1);
+    const int al = max(0, min(int(alias[init] + 0.5f), nc - 1));
+    const int cell = rng.next_1d() < prob[init] ? init : al;
+    const int x = cell % S.tab_w, y = cell / S.tab_w;
+    const float u1 = rng.next_1d();
+    const float u2 = rng.next_1d();
+    float u = (float(x) + u1) / float(S.tab_w);
+    float v = (float(y) + u2) / float(S.tab_h);
+    u = u - floorf(u);
+    v = fminf(fmaxf(v, 0.f), 1.f);
+    const float theta = u * 2.0f * kPiF - kPiF;
+    const float phi = (v - 0.5f) * kPiF;
+    float st, ct, sp, cp;
+    sincosf(theta, &st, &ct);
+    sincosf(phi, &sp, &cp);
+    pdf = pdfs[cell];
+    return env_inverse_rotate(v3(st * cp, ct * cp, -sp));
+}
+
+__device__ float env_pdf(const DeviceScene& S, V3 rd) {
+    if (!has_env_light(S)) return 0.f;
+    const V3 dir = env_rotate(rd);
+    float u, v;
+    equirect_uv(dir, u, v);
+    int x = int(u * float(S.tab_w)) % S.tab_w;
+    if (x < 0) x += S.tab_w;
+    const int y = max(0, min(int(v * float(S.tab_h)), S.tab_h - 1));
+    return S.tab[2 * S.tab_h * S.tab_w + y * S.tab_w + x];
+}
+
+// ---------------------------------------------------------------- BRDF (material.cuh)
+__device__ __forceinline__ float sat(float v) { return fminf(fmaxf(v, 0.f), 1.f); }
+__device__ __forceinline__ V3 sat3(V3 v) { return v3(sat(v.x), sat(v.y), sat(v.z)); }
+__device__ __forceinline__ float clamp_rough(float r) { return fminf(fmaxf(r, kMinRough), 1.f); }
+__device__ __forceinline__ V3 nonneg(V3 v) { return v3(fmaxf(v.x, 0.f), fmaxf(v.y, 0.f), fmaxf(v.z, 0.f)); }
+__device__ __forceinline__ V3 nonneg_mask(V3 v) { return v3(v.x > 0.f ? 1.f : 0.f, v.y > 0.f ? 1.f : 0.f, v.z > 0.f ? 1.f : 0.f); }
+__device__ __forceinline__ V3 fsn(V3 v, V3 fb) {
+    const float l2 = dot(v, v);
+    return l2 > kBrdfEps ? v * rsqrtf(l2) : fb;
+}
+__device__ __forceinline__ V3 to_world(V3 n, V3 l) {
+    V3 t, b;
+    onb(n, t, b);
+    return t * l.x + b * l.y + n * l.z;
+}
+__device__ __forceinline__ V3 sample_diffuse(V3 n, float u1, float u2) {
+    const float xi = sat(u1), r = sqrtf(xi), phi = 2.0f * kPiF * sat(u2);
+    float s, c;
+    sincosf(phi, &s, &c);
+    const float z = sqrtf(fmaxf(0.f, 1.f - xi));
+    return to_world(n, v3(r * c, r * s, z));
+}
+__device__ __forceinline__ V3 sample_ggx_h(V3 n, float rough, float u1, float u2) {
+    const float a = rough * rough, a2 = a * a, xi = sat(u1);
+    const float den = fmaxf(1.f + (a2 - 1.f) * xi, kBrdfEps);
+    const float ct = sqrtf(fmaxf(0.f, (1.f - xi) / den));
+    const float st = sqrtf(fmaxf(0.f, 1.f - ct * ct));
+    float s, c;
+    sincosf(2.0f * kPiF * sat(u2), &s, &c);
+    return to_world(n, v3(st * c, st * s, ct));
+}
+
+struct BG {
+    V3 value, dAlb, dMet, dRough, dDir;
+};
+
+template <bool GRADS>
+__device__ BG eval_ls(V3 wo, V3 n, V3 albedo, float metal, float roughness, V3 ldir, float& pdf, bool metallic) {
+    BG res;
+    res.value = res.dAlb = res.dMet = res.dRough = res.dDir = v3s(0.f);
+    const V3 L = fsn(ldir, n);
+    const float NV = sat(dot(n, wo)), NL = sat(dot(n, L));
+    pdf = 0.f;
+    if (NV <= 0.f || NL <= 0.f) return res;
+    const float rough = clamp_rough(roughness);
+    const V3 f0 = v3s(0.04f) + (albedo - v3s(0.04f)) * metal;
+    const V3 H = fsn(wo + L, n);
+    const float NH = sat(dot(n, H)), VH = sat(dot(wo, H));
+    const float x = 1.f - sat(VH), x2 = x * x, q = x2 * x2 * x;
+    const V3 F = f0 + (v3s(1.f) - f0) * q;
+    const float invPi = 1.0f / kPiF;
+    const V3 diffCol = albedo * (1.f - metal);
+    const float diffScale = NL * invPi;
+    const V3 diffuse = diffCol * (v3s(1.f) - F) * diffScale;
+    const float al = rough * rough, a2 = al * al;
+    const float dDen = fmaxf(NH * NH * (a2 - 1.f) + 1.f, kBrdfEps);
+    const float D = a2 * invPi / fmaxf(dDen * dDen, kBrdfEps);
+    const float k = 0.5f * rough * rough;
+    const float Dv = fmaxf(NV * (1.f - k) + k, kBrdfEps);
+    const float Dl = fmaxf(NL * (1.f - k) + k, kBrdfEps);
+    const float Gv = NV / Dv, Gl = NL / Dl, G = Gv * Gl;
+    const float specScale = D * G / fmaxf(4.f * NV, kBrdfEps);
+    const V3 spec = F * specScale;
+    const float specPdf = VH > 0.f ? D * NH / fmaxf(4.f * VH, kBrdfEps) : 0.f;
+    pdf = 0.5f * (diffScale + specPdf);
+    const V3 value = diffuse + spec;
+    res.value = nonneg(value);
+    if (!GRADS) return res;
+    const V3 mask = nonneg_mask(value);
+    const float dF_dAlb = metal * (1.f - q);
+    if (metallic) {
+        const V3 dF_dMet = (albedo - v3s(0.04f)) * (1.f - q);
+        const V3 dDiff_dMet = (-albedo * (v3s(1.f) - F) - diffCol * dF_dMet) * diffScale;
+        res.dMet = (dDiff_dMet + dF_dMet * specScale) * mask;
+    }
+    const V3 dDiff_dAlb = ((v3s(1.f) - F) - albedo * dF_dAlb) * ((1.f - metal) * diffScale);
+    res.dAlb = (dDiff_dAlb + v3s(dF_dAlb * specScale)) * mask;
+    const float dD_da2 = invPi * (1.f / fmaxf(dDen * dDen, kBrdfEps) - (2.f * a2 * NH * NH) / fmaxf(dDen * dDen * dDen, kBrdfEps));
+    const float dD_dr = dD_da2 * (4.f * rough * rough * rough);
+    const float dGv_dk = -NV * (1.f - NV) / (Dv * Dv);
+    const float dGl_dk = -NL * (1.f - NL) / (Dl * Dl);
+    const float dG_dr = (Gl * dGv_dk + Gv * dGl_dk) * rough;
+    const float dSpec_dr = (dD_dr * G + D * dG_dr) / fmaxf(4.f * NV, kBrdfEps);
+    const float mr = (roughness > kMinRough && roughness < 1.f) ? 1.f : 0.f;
+    res.dRough = F * (dSpec_dr * mr) * mask;
+    return res;
+}
+
+__device__ float pdf_rough_grad(V3 wo, V3 n, float roughness, V3 ldir) {
+    const V3 L = fsn(ldir, n);
+    const float NV = sat(dot(n, wo)), NL = sat(dot(n, L));
+    if (NV <= 0.f || NL <= 0.f) return 0.f;
+    if (!(roughness > kMinRough && roughness < 1.f)) return 0.f;
+    const float rough = clamp_rough(roughness);
+    const V3 H = fsn(wo + L, n);
+    const float NH = sat(dot(n, H)), VH = sat(dot(wo, H));
+    if (VH <= 0.f) return 0.f;
+    const float a2 = rough * rough * rough * rough;
+    const float dDen = fmaxf(NH * NH * (a2 - 1.f) + 1.f, kBrdfEps);
+    const float dD_da2 = (1.0f / kPiF) * (1.f / fmaxf(dDen * dDen, kBrdfEps) - (2.f * a2 * NH * NH) / fmaxf(dDen * dDen * dDen, kBrdfEps));
+    const float dD_dr = dD_da2 * (4.f * rough * rough * rough);
+    return 0.5f * dD_dr * NH / fmaxf(4.f * VH, kBrdfEps);
+}
+
+template <bool GRADS>
+__device__ BG sample_throughput(V3 wo, V3 n, V3 albedo, float metal, float roughness, V3 rnd, V3& L, float& pdf,
+                                bool metallic) {
+    const float rough = clamp_rough(roughness);
+    BG res;
+    res.value = res.dAlb = res.dMet = res.dRough = res.dDir = v3s(0.f);
+    L = n;
+    pdf = 0.f;
+    V3 dDir = v3s(0.f);
+    if (rnd.z < 0.5f) {
+        L = sample_diffuse(n, rnd.x, rnd.y);
+        if (!GRADS && !(dot(n, L) > 0.f)) return res;
+    } else {
+        const V3 H = sample_ggx_h(n, rough, rnd.x, rnd.y);
+        L = H * (2.f * dot(wo, H)) - wo;
+        const bool ok = dot(n, L) > 0.f;
+        if (!GRADS && !ok) return res;
+        if (GRADS && (roughness > kMinRough && roughness < 1.f) && ok) {
+            const float rs = clamp_rough(roughness + kRoughGradEps);
+            const float step = rs - rough;
+            if (step > kBrdfEps) {
+                const V3 Hs = sample_ggx_h(n, rs, rnd.x, rnd.y);
+                const V3 Ls = Hs * (2.f * dot(wo, Hs)) - wo;
+                if (dot(n, Ls) > 0.f) dDir = (Ls - L) / step;
+            }
+        }
+    }
+    if (dot(n, L) <= 0.f) return res;
+    float p;
+    const BG bc = eval_ls<GRADS>(wo, n, albedo, metal, roughness, L, p, metallic);
+    pdf = p;
+    if (p <= kBrdfEps) {
+        pdf = 0.f;
+        return res;
+    }
+    const float invPdf = 1.f / fmaxf(p, kBrdfEps);
+    const V3 value = bc.value * invPdf;
+    res.value = nonneg(value);
+    res.dDir = dDir;
+    if (GRADS) {
+        const V3 mask = nonneg_mask(value);
+        const float dPdf = pdf_rough_grad(wo, n, roughness, L);
+        res.dAlb = bc.dAlb * invPdf * mask;
+        res.dRough = (bc.dRough * invPdf - bc.value * (dPdf * invPdf * invPdf)) * mask;
+        if (metallic) res.dMet = bc.dMet * invPdf * mask;
+    }
+    return res;
+}
+
+__device__ __forceinline__ void shading_frame(const DeviceScene& S, const Seg& s, V3& n, V3& wo, V3& albedo, float& metal) {
+    n = fsn(s.sn, v3(0.f, 0.f, 1.f));
+    wo = fsn(-s.ray.d, n);
+    albedo = sat3(s.alb);
+    metal = S.enable_metallic ? sat(s.metal) : 0.f;
+}
+
+__device__ __forceinline__ float mis(float a, float b) {
+    const float a2 = a * a, b2 = b * b;
+    return a2 / fmaxf(a2 + b2, 1e-6f);
+}
+
+// ---------------------------------------------------------------- segment tracing
+__device__ __forceinline__ bool self_occluded(const DeviceScene& S, int slot, float t, V3 d) {
+    if (t > kMaxSelfOcc || !S.has_snormal) return false;
+    const float4 m1 = S.srec[2 * slot + 1];
+    return dot(v3(m1.y, m1.z, m1.w), d) > kSelfOccDot;
+}
+
+struct HitGeom {
+    V3 gro, grd;
+    float gd, alpha, sig;
+};
+
+__device__ __forceinline__ float hit_alpha(const DeviceScene& S, int slot, const Ray& r, HitGeom& h) {
+    const float4* g = S.grec + 4 * slot;
+    const float4 a0 = g[0], a1 = g[1], a2 = g[2], p = g[3];
+    const V3 dp = r.o - v3(a0.w, a1.w, a2.w);
+    h.gro = v3(dot(xyz(a0), dp), dot(xyz(a1), dp), dot(xyz(a2), dp));
+    h.grd = safe_normalize(v3(dot(xyz(a0), r.d), dot(xyz(a1), r.d), dot(xyz(a2), r.d)));
+    const V3 gc = cross(h.grd, h.gro);
+    h.gd = dot(gc, gc);
+    const float gres = response(S.kernel_degree, h.gd);
+    const float a = fminf(0.99f, gres * p.x);
+    h.sig = p.x;
+    return (gres > S.min_response && a > kAlphaMin) ? a : 0.f;
+}
+
+__device__ __forceinline__ uint32_t orig_id(const DeviceScene& S, int slot) {
+    return uint32_t(__float_as_int(S.grec[4 * slot + 3].z));
+}
+
+__device__ V3 sh_radiance(const DeviceScene& S, uint32_t id, V3 d) {
+    const float* c = S.sh + size_t(S.sh_stride) * id;
+    auto C = [&](int k) { return v3(c[3 * k], c[3 * k + 1], c[3 * k + 2]); };
+    V3 r = 0.28209479177387814f * C(0);
+    const int deg = S.sph_degree;
+    if (deg > 0) {
+        const float x = d.x, y = d.y, z = d.z;
+        const float C1 = 0.4886025119029199f;
+        r = r - (C1 * y) * C(1) + (C1 * z) * C(2) - (C1 * x) * C(3);
+        if (deg > 1) {
+            const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
+            r = r + (1.0925484305920792f * xy) * C(4) + (-1.0925484305920792f * yz) * C(5) +
+                (0.31539156525252005f * (2.f * zz - xx - yy)) * C(6) + (-1.0925484305920792f * xz) * C(7) +
+                (0.5462742152960396f * (xx - yy)) * C(8);
+            if (deg > 2) {
+                r = r + (-0.5900435899266435f * y * (3.f * xx - yy)) * C(9) + (2.890611442640554f * xy * z) * C(10) +
+                    (-0.4570457994644658f * y * (4.f * zz - xx - yy)) * C(11) +
+                    (0.3731763325901154f * z * (2.f * zz - 3.f * xx - 3.f * yy)) * C(12) +
+                    (-0.4570457994644658f * x * (4.f * zz - xx - yy)) * C(13) +
+                    (1.445305721320277f * z * (xx - yy)) * C(14) + (-0.5900435899266435f * x * (xx - 3.f * yy)) * C(15);
+            }
+        }
+    }
+    r = r + v3s(0.5f);
+    return v3(fmaxf(r.x, 0.f), fmaxf(r.y, 0.f), fmaxf(r.z, 0.f));
+}
+
+// pathTracer.cuh:216-311 rayIntersect with gaussianParticles.cuh:343-440 processHit
+template <int K>
+__device__ void ray_intersect(const DeviceScene& S, const Ray ray, Seg& pl, Rng& rng, bool selfOcc, bool primary,
+                              uint32_t* vis, const Lane& L) {
+    float t0, t1;
+    scene_interval(S, ray, t0, t1);
+    const float sPdf = pl.scatterPdf, lPdf = pl.lightPdf;
+    pl.reset(ray, fmaxf(0.f, t0 - kEpsT));
+    pl.scatterPdf = sPdf;
+    pl.lightPdf = lPdf;
+    float depth = 0.f;
+    V3 alb = v3s(0.f), snAcc = v3s(0.f);
+    float rough = 0.f, metal = 0.f;
+    const float minT = S.min_transmittance;
+    const bool need_rad = primary || S.render_opts == 1;
+    const bool need_nrm = primary && S.enable_normals;
+    const float dlen = sqrtf(dot(ray.d, ray.d));
+    count(S, C_SEGMENTS);
+    while (pl.lastHit <= t1 && pl.T > minT) {
+        const int cnt = collect<K>(S, ray, pl.lastHit + kEpsT, t1 + kEpsT, L);
+        if (cnt == 0) break;
+        for (int k = 0; k < cnt; ++k) {
+            if (!(pl.T > minT)) break;
+            const float t = __uint_as_float(L.at(k));
+            const int slot = int(L.at(K + k));
+            count(S, C_CANDIDATES);
+            if (selfOcc && self_occluded(S, slot, t, ray.d)) {
+                pl.lastHit = fmaxf(pl.lastHit, t);
+                continue;
+            }
+            HitGeom h;
+            const float a = hit_alpha(S, slot, ray, h);
+            const bool acc = a > 0.f;
+            if (acc) {
+                const float w = a * pl.T;
+                const float hitT = t * dlen;
+                const uint32_t id = orig_id(S, slot);
+                if (need_rad) pl.radiance += sh_radiance(S, id, ray.d) * w;
+                const float4 m0 = S.srec[2 * slot], m1 = S.srec[2 * slot + 1];
+                alb += xyz(m0) * w;
+                rough += m0.w * w;
+                metal += m1.x * w;
+                pl.T *= (1.f - a);
+                depth += hitT * w;
+                pl.hitDist2 += hitT * hitT * w;
+                if (S.enable_normals) {
+                    snAcc += w * v3(m1.y, m1.z, m1.w);
+                    if (need_nrm) {
+                        const float4 rs = S.prec[slot];
+                        const float proj = dot(h.grd, -h.gro);
+                        pl.normal += w * safe_normalize((h.gro + h.grd * (proj - sqrtf(9.f - h.gd))) * xyz(rs));
+                    }
+                }
+                pl.hit = true;
+                const float op = 1.f - pl.T;
+                pl.distortion = fmaxf(op * pl.hitDist2 - depth * depth, 0.f);
+                if (vis) vis[id] = 1u;
+                count(S, C_ACCEPTED);
+                if (primary) count(S, C_ACCEPTED_PRIMARY);
+            }
+            pl.lastHit = fmaxf(pl.lastHit, t);
+            if (S.enable_hitcounts) pl.hits += acc ? 1.f : 0.f;
+        }
+    }
+    const float op = 1.f - pl.T;
+    pl.hitDist = depth;
+    pl.valid = false;
+    if (pl.hit && op > 0.5f) {
+        const float inv = 1.f / fmaxf(op, 1e-12f);
+        pl.ivalid = true;
+        pl.pos = ray.o + ray.d * (depth * inv);
+        pl.sn = snAcc * inv;
+        pl.alb = alb * inv;
+        pl.rough = rough * inv;
+        pl.metal = metal * inv;
+    } else {
+        const float u = rng.next_1d();
+        pl.valid = u < pl.T;
+    }
+}
+
+// pathTracer.cuh:161-214 stochastic shadow ray
+template <int K>
+__device__ bool shadow_occluded(const DeviceScene& S, const Ray ray, Rng& rng, float tmin, float tmax, const Lane& L) {
+    float t0, t1;
+    scene_interval(S, ray, t0, t1);
+    float startT = fmaxf(tmin, t0 - kEpsT);
+    const float endT = fminf(tmax, t1) + kEpsT;
+    if (startT >= endT) return false;
+    while (startT < endT) {
+        const int cnt = collect<K>(S, ray, startT + kEpsT, endT, L);
+        if (cnt == 0) break;
+        float batchEnd = startT;
+        for (int k = 0; k < cnt; ++k) {
+            const float t = __uint_as_float(L.at(k));
+            const int slot = int(L.at(K + k));
+            if (t > endT) break;
+            batchEnd = fmaxf(batchEnd, t);
+            count(S, C_SHADOW_CANDIDATES);
+            if (self_occluded(S, slot, t, ray.d)) continue;
+            HitGeom h;
+            const float a = hit_alpha(S, slot, ray, h);
+            if (a > 0.f && rng.next_1d() < a) return true;
+        }
+        if (batchEnd <= startT) break;
+        startT = batchEnd;
+    }
+    return false;
+}
+
+// pathTracer.cuh:313-390 material-gradient flush over a replayed segment
+template <int K>
+__device__ void intersect_bwd(const DeviceScene& S, const Ray ray, float opacity, float maxHit, V3 ga, float gr,
+                              float gm, bool selfOcc, float* dmat_leaf, const Lane& L) {
+    const float inv = 1.f / fmaxf(opacity, 1e-12f);
+    ga = ga * inv;
+    gr *= inv;
+    gm *= inv;
+    float t0, t1;
+    scene_interval(S, ray, t0, t1);
+    float startT = fmaxf(0.f, t0 - kEpsT);
+    const float endT = fminf(maxHit, t1) + kEpsT;
+    float T = 1.f;
+    const float minT = S.min_transmittance;
+    while (startT < endT && T > minT) {
+        const int cnt = collect<K>(S, ray, startT + kEpsT, endT, L);
+        if (cnt == 0) break;
+        for (int k = 0; k < cnt; ++k) {
+            if (!(T > minT)) break;
+            const float t = __uint_as_float(L.at(k));
+            const int slot = int(L.at(K + k));
+            if (selfOcc && self_occluded(S, slot, t, ray.d)) {
+                startT = fmaxf(startT, t);
+                continue;
+            }
+            HitGeom h;
+            const float a = hit_alpha(S, slot, ray, h);
+            if (a > 0.f) {
+                const float w = a * T;
+                float* q = dmat_leaf + 5 * size_t(slot);
+                atomicAdd(q + 0, w * ga.x);
+                atomicAdd(q + 1, w * ga.y);
+                atomicAdd(q + 2, w * ga.z);
+                atomicAdd(q + 3, w * gr);
+                if (S.enable_metallic) atomicAdd(q + 4, w * gm);
+                count(S, C_MAT_ATOMICS);
+                T *= (1.f - a);
+            }
+            startT = fmaxf(startT, t);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- path logic
+template <int K>
+__device__ void sample_nee(const DeviceScene& S, Path& p, Rng& rng, const Lane& L) {
+    emit_clear(p);
+    if (!has_env_light(S)) return;
+    float envPdf;
+    const V3 wi = sample_env(S, rng, envPdf);
+    const V3 Li = background(S, wi);
+    if (envPdf <= 0.f || is_zero(Li)) return;
+    if (dot(p.cur.sn, wi) <= 0.f) return;
+    V3 n, wo, alb;
+    float met;
+    shading_frame(S, p.cur, n, wo, alb, met);
+    float sPdf;
+    const BG bc = eval_ls<false>(wo, n, alb, met, clamp_rough(p.cur.rough), wi, sPdf, S.enable_metallic);
+    const Ray lr{p.cur.pos + wi * kSelfOccOffset, wi};
+    if (shadow_occluded<K>(S, lr, rng, 0.f, kInfDist, L)) return;
+    p.eray = lr;
+    p.elightPdf = envPdf;
+    p.escatterPdf = sPdf;
+    p.econtrib = p.thr * bc.value * Li / fmaxf(envPdf, 1e-6f);
+}
+
+__device__ void accumulate(const DeviceScene& S, Path& p) {
+    p.cur.contribution = v3s(0.f);
+    const bool first = p.numBounces == 1u;
+    const bool mis_on = S.enable_mis != 0;
+    const float bMis = (mis_on && first) ? mis(p.cur.scatterPdf, p.cur.lightPdf) : 1.f;
+    const float lMis = (mis_on && first) ? mis(p.elightPdf, p.escatterPdf) : 1.f;
+    const V3 nee = p.econtrib * lMis;
+    if (S.render_opts == 1) {
+        if (mis_on && first) p.L += nee;
+        if (p.numBounces > 0u && p.cur.ivalid) {
+            p.cur.contribution = p.thr * p.cur.radiance;
+            if (mis_on && first) p.cur.contribution = p.cur.contribution * bMis;
+            p.L += p.cur.contribution;
+            p.indirect += p.cur.contribution;
+        } else if (p.cur.valid) {
+            if (!has_env(S)) return;
+            const V3 bg = background(S, p.cur.ray.d);
+            p.thr = p.thr * p.cur.T;
+            p.cur.light += bg;
+            p.Lnb += bg;
+            p.cur.contribution = p.thr * p.cur.light;
+            if (mis_on && first) p.cur.contribution = p.cur.contribution * bMis;
+            p.L += p.cur.contribution;
+            p.direct += p.cur.contribution;
+        }
+        return;
+    }
+    if (mis_on && first) {
+        p.L += nee;
+        p.direct += nee;
+    }
+    if (p.cur.valid) {
+        if (!has_env(S)) return;
+        const V3 bg = background(S, p.cur.ray.d);
+        p.thr = p.thr * p.cur.T;
+        p.cur.light += bg;
+        p.Lnb += bg;
+        p.cur.contribution = p.thr * p.cur.light;
+        if (mis_on && first) p.cur.contribution = p.cur.contribution * bMis;
+        p.cur.radiance += p.cur.contribution;
+        p.L += p.cur.contribution;
+        if (p.numBounces < 2u) p.direct += p.cur.contribution;
+        else p.indirect += p.cur.contribution;
+    }
+}
+
+__device__ void accumulate_bwd(const DeviceScene& S, Path& p, float* denv) {
+    p.cur.contribution = v3s(0.f);
+    const bool first = p.numBounces == 1u;
+    const bool mis_on = S.enable_mis != 0;
+    const float bMis = (mis_on && first) ? mis(p.cur.scatterPdf, p.cur.lightPdf) : 1.f;
+    const float lMis = (mis_on && first) ? mis(p.elightPdf, p.escatterPdf) : 1.f;
+    if (S.render_opts == 1) {
+        if (mis_on && first) p.L -= p.econtrib * lMis;
+        if (p.numBounces > 0u && p.cur.ivalid) {
+            p.cur.contribution = p.thr * p.cur.radiance;
+            if (mis_on && first) p.cur.contribution = p.cur.contribution * bMis;
+            p.L -= p.cur.contribution;
+        } else if (p.cur.valid) {
+            if (!has_env(S)) return;
+            p.thr = p.thr * p.cur.T;
+            V3 eg = p.dL * p.thr;
+            if (mis_on && first) eg = eg * bMis;
+            eg += p.dLnb;
+            const V3 vis = background_bwd(S, p.cur.ray.d, eg, p.cur.dirGrad, denv);
+            p.cur.contribution = p.thr * vis;
+            if (mis_on && first) p.cur.contribution = p.cur.contribution * bMis;
+            p.L -= p.cur.contribution;
+        }
+        return;
+    }
+    if (mis_on && first) p.L -= p.econtrib * lMis;
+    if (p.cur.valid) {
+        if (!has_env(S)) return;
+        p.thr = p.thr * p.cur.T;
+        V3 eg = p.dL * p.thr;
+        if (mis_on && first) eg = eg * bMis;
+        eg += p.dLnb;
+        const V3 vis = background_bwd(S, p.cur.ray.d, eg, p.cur.dirGrad, denv);
+        p.cur.contribution = p.thr * vis;
+        if (mis_on && first) p.cur.contribution = p.cur.contribution * bMis;
+        p.cur.radiance += p.cur.contribution;
+        p.L -= p.cur.contribution;
+    }
+}
+
+__device__ void sample_next(const DeviceScene& S, Path& p, Rng& rng) {
+    V3 n, wo, alb;
+    float met;
+    shading_frame(S, p.cur, n, wo, alb, met);
+    const float r0 = rng.next_1d();
+    const float r1 = rng.next_1d();
+    const float r2 = rng.next_1d();
+    V3 L = p.cur.ray.d;
+    float sPdf = 0.f;
+    const BG bt = sample_throughput<false>(wo, n, alb, met, clamp_rough(p.cur.rough), v3(r0, r1, r2), L, sPdf,
+                                           S.enable_metallic);
+    const V3 nt = p.thr * bt.value;
+    if (is_zero(nt)) {
+        p.thr = v3s(0.f);
+        p.active = 0;
+        return;
+    }
+    p.thr = nt;
+    const Ray nr{p.cur.pos + safe_normalize(L) * kSelfOccOffset, L};
+    const float lpdf = S.enable_mis ? env_pdf(S, L) : 0.f;
+    p.cur.reset(nr, 0.f);
+    if (S.enable_mis) {
+        p.cur.scatterPdf = sPdf;
+        p.cur.lightPdf = lpdf;
+    }
+}
+
+template <int K>
+__device__ void pending_bwd(const DeviceScene& S, Path& p, float* dmat, const Lane& L) {
+    if (!p.pvalid) return;
+    const float dR = dot(p.cur.dirGrad, p.pdir);
+    p.pvalid = false;
+    if (dR == 0.f) return;
+    intersect_bwd<K>(S, p.pray, p.popacity, p.pmaxHit, v3s(0.f), dR, 0.f, p.pbounces > 1u, dmat, L);
+}
+
+__device__ void nee_grad_bwd(const DeviceScene& S, Path& p, const Seg& cs) {
+    if (p.numBounces != 1u) return;
+    if (p.elightPdf <= 0.f) return;
+    V3 n, wo, alb;
+    float met;
+    shading_frame(S, cs, n, wo, alb, met);
+    float sPdf;
+    const BG bc = eval_ls<true>(wo, n, alb, met, cs.rough, p.eray.d, sPdf, S.enable_metallic);
+    const float lMis = mis(p.elightPdf, p.escatterPdf);
+    const float scale = lMis / fmaxf(p.elightPdf, 1e-6f);
+    const V3 bg = background(S, p.eray.d);
+    const V3 dLdB = p.dL * p.thr * bg * scale;
+    p.cur.galb += dLdB * bc.dAlb;
+}
+
+template <int K>
+__device__ void sample_next_bwd(const DeviceScene& S, Path& p, Rng& rng, float* dmat, const Lane& L) {
+    V3 n, wo, alb;
+    float met;
+    shading_frame(S, p.cur, n, wo, alb, met);
+    const float r0 = rng.next_1d();
+    const float r1 = rng.next_1d();
+    const float r2 = rng.next_1d();
+    V3 Ld = p.cur.ray.d;
+    float sPdf = 0.f;
+    const BG bt = sample_throughput<true>(wo, n, alb, met, p.cur.rough, v3(r0, r1, r2), Ld, sPdf, S.enable_metallic);
+    const V3 nt = p.thr * bt.value;
+    if (is_zero(nt)) {
+        p.thr = v3s(0.f);
+        p.active = 0;
+        return;
+    }
+    const bool mis_on = S.enable_mis != 0;
+    float lPdf = 0.f;
+    bool firstNee = false;
+    if (mis_on) {
+        lPdf = env_pdf(S, Ld);
+        firstNee = p.numBounces == 1u && vmax(nt) >= 1e-4f;
+        if (firstNee) nee_grad_bwd(S, p, p.cur);
+    }
+    V3 dep = p.L;
+    if (mis_on && firstNee) dep -= p.econtrib * mis(p.elightPdf, p.escatterPdf);
+    const V3 num = p.dL * dep;
+    const V3 dLdB = v3(bt.value.x > kBrdfEps ? num.x / bt.value.x : 0.f, bt.value.y > kBrdfEps ? num.y / bt.value.y : 0.f,
+                       bt.value.z > kBrdfEps ? num.z / bt.value.z : 0.f);
+    p.cur.galb += dLdB * bt.dAlb;
+    p.cur.grough += dot(dLdB, bt.dRough);
+    if (S.enable_metallic) p.cur.gmetal += dot(dLdB, bt.dMet);
+    p.pvalid = dot(bt.dDir, bt.dDir) > 0.f;
+    p.pray = p.cur.ray;
+    p.popacity = 1.f - p.cur.T;
+    p.pmaxHit = p.cur.lastHit;
+    p.pdir = bt.dDir;
+    p.pbounces = p.numBounces;
+    intersect_bwd<K>(S, p.cur.ray, 1.f - p.cur.T, p.cur.lastHit, p.cur.galb, p.cur.grough, p.cur.gmetal,
+                     p.numBounces > 1u, dmat, L);
+    p.thr = nt;
+    const Ray nr{p.cur.pos + safe_normalize(Ld) * kSelfOccOffset, Ld};
+    p.cur.reset(nr, 0.f);
+    if (mis_on) {
+        p.cur.scatterPdf = sPdf;
+        p.cur.lightPdf = lPdf;
+    }
+}
+
+// ---------------------------------------------------------------- launch mapping
+struct PixelId {
+    int x, y, z;
+    bool ok;
+};
+
+// Block b of the launch runs on XCD (b mod 8); give each XCD a contiguous range of tiles.
+__device__ __forceinline__ PixelId pixel_of(const DeviceScene& S) {
+    const int nb = S.num_blocks;
+    const int b = int(blockIdx.x);
+    const int per = nb / 8, rem = nb % 8;
+    const int xcd = b % 8, k = b / 8;
+    const int tile = (xcd < rem) ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
+    const int tpi = S.tiles_x * S.tiles_y;
+    const int z = tile / tpi;
+    const int r = tile - z * tpi;
+    const int ty = r / S.tiles_x, tx = r - ty * S.tiles_x;
+    const int lane = int(threadIdx.x) & 63, wave = int(threadIdx.x) >> 6;
+    PixelId p;
+    p.x = tx * 16 + wave * 8 + (lane & 7);
+    p.y = ty * 8 + (lane >> 3);
+    p.z = z;
+    p.ok = p.x < S.W && p.y < S.H && z < S.B;
+    return p;
+}
+
+__device__ __forceinline__ Ray primary_ray(const DeviceScene& S, size_t pix) {
+    const float* M = S.ray_to_world;
+    const float* o = S.ray_ori + 3 * pix;
+    const float* d = S.ray_dir + 3 * pix;
+    const float ox = o[0], oy = o[1], oz = o[2], dx = d[0], dy = d[1], dz = d[2];
+    Ray r;
+    r.o = v3(M[0] * ox + M[1] * oy + M[2] * oz + M[3], M[4] * ox + M[5] * oy + M[6] * oz + M[7],
+             M[8] * ox + M[9] * oy + M[10] * oz + M[11]);
+    r.d = v3(M[0] * dx + M[1] * dy + M[2] * dz, M[4] * dx + M[5] * dy + M[6] * dz, M[8] * dx + M[9] * dy + M[10] * dz);
+    return r;
+}
+
+__device__ __forceinline__ void path_init(Path& p) {
+    p.active = 1u;
+    p.numBounces = 0u;
+    emit_clear(p);
+    p.L = p.dL = p.direct = p.indirect = p.Lnb = p.dLnb = v3s(0.f);
+    p.thr = v3s(1.f);
+    p.pvalid = false;
+    p.pbounces = 0u;
+    p.popacity = p.pmaxHit = 0.f;
+    p.pdir = v3s(0.f);
+    p.cur.reset(Ray{v3s(0.f), v3s(0.f)}, 0.f);
+}
+
+// referenceOptix.cu:27-71
+template <int K>
+__global__ void __launch_bounds__(kTraceBlock) k_trace_fwd(DeviceScene S, FwdOut out) {
+    __shared__ uint32_t lds[LdsLayout<K>::kWords * kTraceBlock];
+    const Lane L{lds + threadIdx.x};
+    const PixelId px = pixel_of(S);
+    if (!px.ok) return;
+    const size_t pix = (size_t(px.z) * S.H + px.y) * S.W + px.x;
+    Rng rng;
+    rng.seed(Rng::launch_seed(uint32_t(px.x), uint32_t(px.y), uint32_t(px.z), S.frame));
+    Path p;
+    path_init(p);
+    count(S, C_PATHS);
+    ray_intersect<K>(S, primary_ray(S, pix), p.cur, rng, false, true, out.vis, L);
+    {
+        const Seg& c = p.cur;
+        float* o;
+        o = out.rad + 3 * pix; o[0] = c.radiance.x; o[1] = c.radiance.y; o[2] = c.radiance.z;
+        out.dns[pix] = 1.f - c.T;
+        o = out.hit + 2 * pix; o[0] = c.hitDist; o[1] = c.lastHit;
+        out.hit2[pix] = c.hitDist2;
+        out.dist[pix] = c.distortion;
+        o = out.nrm + 3 * pix; o[0] = c.normal.x; o[1] = c.normal.y; o[2] = c.normal.z;
+        o = out.snrm + 3 * pix; o[0] = c.sn.x; o[1] = c.sn.y; o[2] = c.sn.z;
+        o = out.mat + 5 * pix; o[0] = c.alb.x; o[1] = c.alb.y; o[2] = c.alb.z; o[3] = c.rough; o[4] = c.metal;
+        out.hits[pix] = c.hits;
+    }
+    if (p.cur.ivalid) {
+        if (S.enable_mis) sample_nee<K>(S, p, rng, L);
+        for (unsigned depth = 0; depth < S.max_bounces && p.active; ++depth) {
+            accumulate(S, p);
+            p.active &= (depth + 1u < S.max_bounces) && p.cur.ivalid;
+            if (!p.active) break;
+            p.numBounces = depth + 1u;
+            sample_next(S, p, rng);
+            if (vmax(p.thr) < 1e-4f) break;
+            ray_intersect<K>(S, p.cur.ray, p.cur, rng, true, false, out.vis, L);
+        }
+    }
+    float* o;
+    o = out.pbr + 3 * pix; o[0] = p.L.x; o[1] = p.L.y; o[2] = p.L.z;
+    o = out.light + 3 * pix; o[0] = p.Lnb.x; o[1] = p.Lnb.y; o[2] = p.Lnb.z;
+    o = out.comp + 6 * pix;
+    o[0] = p.direct.x; o[1] = p.direct.y; o[2] = p.direct.z;
+    o[3] = p.indirect.x; o[4] = p.indirect.y; o[5] = p.indirect.z;
+}
+
+// referenceBwdOptix.cu:27-74
+template <int K>
+__global__ void __launch_bounds__(kTraceBlock) k_trace_bwd(DeviceScene S, BwdIO io) {
+    __shared__ uint32_t lds[LdsLayout<K>::kWords * kTraceBlock];
+    const Lane L{lds + threadIdx.x};
+    const PixelId px = pixel_of(S);
+    if (!px.ok) return;
+    const size_t pix = (size_t(px.z) * S.H + px.y) * S.W + px.x;
+    Rng rng;
+    rng.seed(Rng::launch_seed(uint32_t(px.x), uint32_t(px.y), uint32_t(px.z), S.frame));
+    Path p;
+    path_init(p);
+    count(S, C_PATHS);
+    p.L = v3(io.pbr[3 * pix], io.pbr[3 * pix + 1], io.pbr[3 * pix + 2]);
+    p.dL = v3(io.pbr_grad[3 * pix], io.pbr_grad[3 * pix + 1], io.pbr_grad[3 * pix + 2]);
+    p.Lnb = v3(io.light[3 * pix], io.light[3 * pix + 1], io.light[3 * pix + 2]);
+    if (io.light_grad) p.dLnb = v3(io.light_grad[3 * pix], io.light_grad[3 * pix + 1], io.light_grad[3 * pix + 2]);
+    ray_intersect<K>(S, primary_ray(S, pix), p.cur, rng, false, true, nullptr, L);
+    if (!p.cur.ivalid) return;
+    if (io.mat_grad) {
+        const float* mg = io.mat_grad + 5 * pix;
+        p.cur.galb += v3(mg[0], mg[1], mg[2]);
+        p.cur.grough += mg[3];
+        if (S.enable_metallic) p.cur.gmetal += mg[4];
+    }
+    if (S.enable_mis) sample_nee<K>(S, p, rng, L);
+    for (unsigned depth = 0; depth < S.max_bounces && p.active; ++depth) {
+        accumulate_bwd(S, p, io.denv);
+        pending_bwd<K>(S, p, io.dmat_leaf, L);
+        p.active &= (depth + 1u < S.max_bounces) && p.cur.ivalid;
+        if (!p.active) break;
+        p.numBounces = depth + 1u;
+        sample_next_bwd<K>(S, p, rng, io.dmat_leaf, L);
+        if (vmax(p.thr) < 1e-4f) break;
+        ray_intersect<K>(S, p.cur.ray, p.cur, rng, true, false, nullptr, L);
+    }
+}
+
+static void tiling(DeviceScene& S) {
+    S.tiles_x = (S.W + 15) / 16;
+    S.tiles_y = (S.H + 7) / 8;
+    S.num_blocks = S.tiles_x * S.tiles_y * S.B;
+}
+
+hipError_t launch_trace_fwd(const DeviceScene& S0, const FwdOut& out, int kbuf, hipStream_t s) {
+    DeviceScene S = S0;
+    tiling(S);
+    if (S.num_blocks == 0) return hipSuccess;
+    switch (kbuf) {
+    case 8: k_trace_fwd<8><<<S.num_blocks, kTraceBlock, 0, s>>>(S, out); break;
+    case 32: k_trace_fwd<32><<<S.num_blocks, kTraceBlock, 0, s>>>(S, out); break;
+    default: k_trace_fwd<16><<<S.num_blocks, kTraceBlock, 0, s>>>(S, out); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_trace_bwd(const DeviceScene& S0, const BwdIO& io, int kbuf, hipStream_t s) {
+    DeviceScene S = S0;
+    tiling(S);
+    if (S.num_blocks == 0) return hipSuccess;
+    switch (kbuf) {
+    case 8: k_trace_bwd<8><<<S.num_blocks, kTraceBlock, 0, s>>>(S, io); break;
+    case 32: k_trace_bwd<32><<<S.num_blocks, kTraceBlock, 0, s>>>(S, io); break;
+    default: k_trace_bwd<16><<<S.num_blocks, kTraceBlock, 0, s>>>(S, io); break;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace ptir

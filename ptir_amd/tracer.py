@@ -1,0 +1,596 @@
+"""Python surface of the MI355X tracer, mirroring threedgptir_tracer/tracer.py.
+
+* `HipTracer` has the method set of the reference's pybind class `OptixTracer`
+  (threedgptir_tracer/bindings.cpp:32-38, include/3dgptir/optixTracer.h:130-217):
+  ``trace`` (13 outputs), ``trace_bwd`` (5 outputs) and ``build_bvh``, same argument
+  order, same output shapes.  It calls the C ABI of libptir_amd.so through ctypes with
+  the current torch stream; outputs are allocated here with torch.
+* `Tracer` keeps the reference's `Tracer(conf)` / `build_acc` / `render` surface and its
+  `_Autograd` function (tracer.py:136-384,428-975), so a PTIR model can swap it in.
+
+Errors raise `PtirError` (the reference swallows CUDA/OptiX errors,
+include/3dgptir/cuoptixMacros.h:22-45).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import logging
+import math
+from dataclasses import dataclass
+from enum import IntEnum
+
+import torch
+
+from . import _lib
+from .filters import Filter, post_processing
+
+logger = logging.getLogger(__name__)
+
+
+def _ptr(t: torch.Tensor | None) -> int | None:
+    return None if t is None else t.data_ptr()
+
+
+def _stream() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _need(t: torch.Tensor, name: str, shape_tail=None, dtype=torch.float32) -> torch.Tensor:
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch.Tensor")
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a device tensor")
+    if t.dtype != dtype:
+        raise TypeError(f"{name} must be {dtype}, got {t.dtype}")
+    if shape_tail is not None and tuple(t.shape[-len(shape_tail):]) != tuple(shape_tail):
+        raise ValueError(f"{name} must have trailing shape {tuple(shape_tail)}, got {tuple(t.shape)}")
+    return t.contiguous()
+
+
+class HipTracer:
+    """Drop-in for the reference's `OptixTracer` pybind class."""
+
+    def __init__(self, path: str = "", cuda_path: str = "", pipeline: str = "reference",
+                 backward_pipeline: str = "referenceBwd", primitive: str = "instances",
+                 particle_kernel_degree: float = 4.0, particle_kernel_min_response: float = 0.0113,
+                 particle_kernel_density_clamping: bool = True, particle_radiance_sph_degree: int = 3,
+                 enable_normals: bool = True, enable_hitcounts: bool = True, enable_mis: bool = True,
+                 enable_metallic: bool = False, visualize_lights: bool = False, kbuffer: int = 16):
+        if pipeline != "reference" or backward_pipeline not in ("referenceBwd", "reference" + "Bwd"):
+            raise ValueError(f"unsupported pipeline pair ({pipeline}, {backward_pipeline}); "
+                             "only the PTIR 'reference' path tracer is built")
+        if primitive != "instances":
+            raise ValueError(f"unsupported primitive type {primitive!r}; only 'instances' is built")
+        o = _lib.Options(float(particle_kernel_degree), float(particle_kernel_min_response),
+                         int(bool(particle_kernel_density_clamping)), int(particle_radiance_sph_degree),
+                         int(bool(enable_normals)), int(bool(enable_hitcounts)), int(bool(enable_mis)),
+                         int(bool(enable_metallic)), int(bool(visualize_lights)), int(kbuffer))
+        h = C.c_void_p()
+        L = _lib.lib()
+        _lib.check(L.ptir_create(C.byref(o), C.byref(h)), None)
+        self._h = h
+        self.options = o
+        self.sph_max_degree = int(particle_radiance_sph_degree)
+        self.last_counters: dict | None = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            _lib.lib().ptir_destroy(h)
+            self._h = None
+
+    # -------------------------------------------------------------- build_bvh
+    def build_bvh(self, mog_pos, mog_rot, mog_scl, mog_dns, rebuild=True, allow_update=False):
+        """OptixTracer::buildBVH (optixTracer.cpp:633-907)."""
+        pos = _need(mog_pos, "mog_pos", (3,))
+        n = pos.shape[0]
+        rot = _need(mog_rot, "mog_rot", (4,))
+        scl = _need(mog_scl, "mog_scl", (3,))
+        dns = _need(mog_dns, "mog_dns").reshape(-1)
+        if rot.shape[0] != n or scl.shape[0] != n or dns.shape[0] != n:
+            raise ValueError("position, rotation, scale and density must have the same number of Gaussians")
+        _lib.check(_lib.lib().ptir_build_bvh(self._h, _ptr(pos), _ptr(rot), _ptr(scl), _ptr(dns), n, int(bool(rebuild)),
+                                             int(bool(allow_update)), _stream()), self._h)
+
+    # -------------------------------------------------------------- shared
+    def _scene(self, particle_density, particle_material, particle_radiance, particle_shading_normal, environment,
+               environment_alias_table, lights, light_alias_table, mesh_lights):
+        pd = _need(particle_density, "particle_density", (12,))
+        n = pd.shape[0]
+        pm = _need(particle_material, "particle_material", (5,))
+        pr = _need(particle_radiance, "particle_radiance")
+        sn = _need(particle_shading_normal, "particle_shading_normal", (3,))
+        if pm.shape[0] != n or pr.shape[0] != n or sn.shape[0] != n:
+            raise ValueError("particle tensors disagree on the number of Gaussians")
+        keep = [pd, pm, pr, sn]
+        sc = _lib.Scene()
+        sc.num_gaussians = n
+        sc.particle_density, sc.particle_material, sc.particle_radiance = _ptr(pd), _ptr(pm), _ptr(pr)
+        sc.sh_stride = pr.shape[1] if pr.dim() > 1 else 0
+        sc.shading_normal = _ptr(sn)
+        env = environment
+        if env is not None and env.numel() > 0 and env.dim() >= 3 and env.shape[0] > 0 and env.shape[1] > 0:
+            env = _need(env, "environment", (4,))
+            keep.append(env)
+            sc.environment, sc.env_h, sc.env_w = _ptr(env), env.shape[0], env.shape[1]
+        else:
+            env = None
+        tab = environment_alias_table
+        if tab is not None and tab.dim() == 3 and tab.shape[0] == 3 and tab.numel() > 0:
+            tab = _need(tab, "environment_alias_table")
+            keep.append(tab)
+            sc.env_table, sc.tab_h, sc.tab_w = _ptr(tab), tab.shape[1], tab.shape[2]
+        sc.num_lights = 0 if lights is None else lights.shape[0]
+        sc.num_light_entries = 0 if light_alias_table is None or light_alias_table.dim() < 2 else light_alias_table.shape[1]
+        sc.num_mesh_lights = 0 if mesh_lights is None else mesh_lights.shape[0]
+        return sc, keep, env
+
+    @staticmethod
+    def _launch(frame, ray_to_world, ray_ori, ray_dir, render_opts, sph_degree, min_transmittance, max_bounces):
+        ro = _need(ray_ori, "ray_ori", (3,))
+        rd = _need(ray_dir, "ray_dir", (3,))
+        if ro.dim() != 4 or ro.shape != rd.shape:
+            raise ValueError(f"rays must be [B,H,W,3] with equal shapes, got {tuple(ro.shape)} and {tuple(rd.shape)}")
+        T = ray_to_world
+        if not T.is_cuda:
+            T = T.to(ro.device)
+        T = _need(T.float(), "ray_to_world")
+        if T.numel() < 12:
+            raise ValueError("ray_to_world must hold at least 12 floats ([B,4,4])")
+        la = _lib.Launch()
+        la.frame, la.ray_to_world, la.ray_ori, la.ray_dir = int(frame) & 0xFFFFFFFF, _ptr(T), _ptr(ro), _ptr(rd)
+        la.B, la.H, la.W = ro.shape[0], ro.shape[1], ro.shape[2]
+        la.render_opts, la.sph_degree = int(render_opts), int(sph_degree)
+        la.min_transmittance, la.max_bounces = float(min_transmittance), int(max_bounces)
+        return la, [ro, rd, T]
+
+    # -------------------------------------------------------------- trace
+    def trace(self, frame_id, ray_to_world, ray_ori, ray_dir, particle_density, particle_material,
+              particle_radiance, particle_shading_normal, environment, environment_alias_table, lights=None,
+              light_alias_table=None, mesh_light_vertices=None, mesh_light_triangles=None, mesh_lights=None,
+              mesh_light_triangle_alias_table=None, render_opts=0, sph_degree=3, min_transmittance=1e-3,
+              max_bounces=3, counters=False):
+        """OptixTracer::trace (optixTracer.cpp:909-1013): returns the 13 outputs
+        (rad, dns, hit[2], hit2, distortion, nrm, snrm, material, hits, visibility, pbr,
+        light, pbr_components)."""
+        sc, keep, _ = self._scene(particle_density, particle_material, particle_radiance, particle_shading_normal,
+                                  environment, environment_alias_table, lights, light_alias_table, mesh_lights)
+        la, keep2 = self._launch(frame_id, ray_to_world, ray_ori, ray_dir, render_opts, sph_degree,
+                                 min_transmittance, max_bounces)
+        B, H, W = la.B, la.H, la.W
+        dev = ray_dir.device
+        f = dict(dtype=torch.float32, device=dev)
+        rad = torch.empty(B, H, W, 3, **f)
+        dns = torch.empty(B, H, W, 1, **f)
+        hit = torch.empty(B, H, W, 2, **f)
+        hit2 = torch.empty(B, H, W, 1, **f)
+        dist = torch.empty(B, H, W, 1, **f)
+        nrm = torch.empty(B, H, W, 3, **f)
+        snrm = torch.empty(B, H, W, 3, **f)
+        mat = torch.empty(B, H, W, 5, **f)
+        hits = torch.empty(B, H, W, 1, **f)
+        vis = torch.zeros(sc.num_gaussians, 1, **f)
+        pbr = torch.empty(B, H, W, 3, **f)
+        light = torch.empty(B, H, W, 3, **f)
+        comp = torch.empty(B, H, W, 2, 3, **f)
+        out = _lib.FwdOutputs(*[_ptr(t) for t in (rad, dns, hit, hit2, dist, nrm, snrm, mat, hits, vis, pbr, light, comp)])
+        cnt = _lib.Counters() if counters else None
+        st = _lib.lib().ptir_trace(self._h, C.byref(sc), C.byref(la), C.byref(out),
+                                   C.byref(cnt) if cnt is not None else None, _stream())
+        _lib.check(st, self._h)
+        del keep, keep2
+        if cnt is not None:
+            self.last_counters = cnt.as_dict()
+        return rad, dns, hit, hit2, dist, nrm, snrm, mat, hits, vis, pbr, light, comp
+
+    # -------------------------------------------------------------- trace_bwd
+    def trace_bwd(self, frame_id, ray_to_world, ray_ori, ray_dir, ray_rad, ray_dns, ray_hit, ray_hit2, ray_dist,
+                  ray_nrm, ray_snrm, ray_mat, ray_pbr, ray_light, particle_density, particle_material,
+                  particle_radiance, particle_shading_normal, environment, environment_alias_table, lights,
+                  light_alias_table, mesh_light_vertices, mesh_light_triangles, mesh_lights,
+                  mesh_light_triangle_alias_table, ray_rad_grd, ray_dns_grd, ray_hit_grd, ray_hit2_grd, ray_dist_grd,
+                  ray_nrm_grd, ray_snrm_grd, ray_mat_grd, ray_pbr_grd, ray_light_grd, render_opts=0, sph_degree=3,
+                  min_transmittance=1e-3, max_bounces=3, counters=False):
+        """OptixTracer::traceBwd (optixTracer.cpp:1015-1145): returns (dDensity [N,12],
+        dMaterial [N,5], dSH, dShadingNormal [N,3], dEnv [H,W,4])."""
+        sc, keep, env = self._scene(particle_density, particle_material, particle_radiance, particle_shading_normal,
+                                    environment, environment_alias_table, lights, light_alias_table, mesh_lights)
+        la, keep2 = self._launch(frame_id, ray_to_world, ray_ori, ray_dir, render_opts, sph_degree,
+                                 min_transmittance, max_bounces)
+        B, H, W = la.B, la.H, la.W
+
+        def pix(t, name, c, allow_none=False):
+            if t is None:
+                if allow_none:
+                    return None
+                raise ValueError(f"{name} is required")
+            t = _need(t, name)
+            if tuple(t.shape) != (B, H, W, c):
+                raise ValueError(f"{name} must be [{B},{H},{W},{c}], got {tuple(t.shape)}")
+            return t
+
+        pbr = pix(ray_pbr, "ray_pbr", 3)
+        light = pix(ray_light, "ray_light", 3)
+        pbr_g = pix(ray_pbr_grd, "ray_pbr_grd", 3)
+        light_g = pix(ray_light_grd, "ray_light_grd", 3, True)
+        mat_g = pix(ray_mat_grd, "ray_mat_grd", 5, True)
+        n = sc.num_gaussians
+        f = dict(dtype=torch.float32, device=ray_dir.device)
+        d_dns = torch.empty(n, particle_density.shape[1], **f)
+        d_mat = torch.empty(n, particle_material.shape[1], **f)
+        d_sh = torch.empty(n, particle_radiance.shape[1], **f)
+        d_sn = torch.empty(n, particle_shading_normal.shape[1], **f)
+        d_env = torch.empty_like(environment) if environment is not None else torch.zeros(0, 0, 4, **f)
+        bi = _lib.BwdInputs(_ptr(pbr), _ptr(light), _ptr(pbr_g), _ptr(light_g), _ptr(mat_g))
+        bo = _lib.BwdOutputs(_ptr(d_dns), _ptr(d_mat), _ptr(d_sh), _ptr(d_sn), _ptr(d_env) if env is not None else None)
+        if env is None and d_env.numel():
+            d_env.zero_()
+        cnt = _lib.Counters() if counters else None
+        st = _lib.lib().ptir_trace_bwd(self._h, C.byref(sc), C.byref(la), C.byref(bi), C.byref(bo),
+                                       C.byref(cnt) if cnt is not None else None, _stream())
+        _lib.check(st, self._h)
+        del keep, keep2
+        if cnt is not None:
+            self.last_counters = cnt.as_dict()
+        return d_dns, d_mat, d_sh, d_sn, d_env
+
+
+# ----------------------------------------------------------------------------- light buffers
+@dataclass(frozen=True)
+class LightBuffers:
+    """Packed scene lights (tracer.py:40-119).  Only empty buffers are accepted by the
+    tracer; point/sphere/mesh lights raise PTIR_ERR_UNSUPPORTED."""
+
+    lights: torch.Tensor
+    alias_table: torch.Tensor
+    mesh_vertices: torch.Tensor
+    mesh_triangles: torch.Tensor
+    mesh_lights: torch.Tensor
+    mesh_triangle_alias_table: torch.Tensor
+
+    @classmethod
+    def empty(cls, device) -> "LightBuffers":
+        f = dict(dtype=torch.float32, device=device)
+        return cls(torch.empty((0, 9), **f), torch.empty((5, 0), **f), torch.empty((0, 3), **f),
+                   torch.empty((0, 3), dtype=torch.int32, device=device), torch.empty((0, 8), **f),
+                   torch.empty((3, 0), **f))
+
+    @classmethod
+    def from_batch(cls, batch, device) -> "LightBuffers":
+        e = cls.empty(device)
+
+        def get(name, fb):
+            v = getattr(batch, name, None)
+            return fb if v is None else v
+
+        return cls(get("lights", e.lights), get("light_alias_table", e.alias_table),
+                   get("mesh_light_vertices", e.mesh_vertices), get("mesh_light_triangles", e.mesh_triangles),
+                   get("mesh_lights", e.mesh_lights),
+                   get("mesh_light_triangle_alias_table", e.mesh_triangle_alias_table)).to(device)
+
+    def to(self, device) -> "LightBuffers":
+        return LightBuffers(self.lights.to(device=device, dtype=torch.float32).contiguous(),
+                            self.alias_table.to(device=device, dtype=torch.float32).contiguous(),
+                            self.mesh_vertices.to(device=device, dtype=torch.float32).contiguous(),
+                            self.mesh_triangles.to(device=device, dtype=torch.int32).contiguous(),
+                            self.mesh_lights.to(device=device, dtype=torch.float32).contiguous(),
+                            self.mesh_triangle_alias_table.to(device=device, dtype=torch.float32).contiguous())
+
+    def tensors(self):
+        return (self.lights, self.alias_table, self.mesh_vertices, self.mesh_triangles, self.mesh_lights,
+                self.mesh_triangle_alias_table)
+
+
+def _conf_get(node, key, default=None):
+    if node is None:
+        return default
+    if isinstance(node, dict):
+        return node.get(key, default)
+    getter = getattr(node, "get", None)
+    if callable(getter):
+        try:
+            return getter(key, default)
+        except TypeError:
+            pass
+    return getattr(node, key, default)
+
+
+# ----------------------------------------------------------------------------- Tracer
+class Tracer:
+    class _Autograd(torch.autograd.Function):
+        """tracer.py:137-384: packs particle tensors, calls trace / trace_bwd and splits
+        the gradients back onto the parameters."""
+
+        @staticmethod
+        def forward(ctx, tracer_wrapper, frame_id, ray_to_world, ray_ori, ray_dir, mog_pos, mog_rot, mog_scl,
+                    mog_dns, mog_sph, mog_snrm, mog_malb, mog_mrgh, mog_mmet, environment, environment_alias_table,
+                    light_buffers, render_opts, sph_degree, min_transmittance, max_bounces):
+            particle_density = torch.concat([mog_pos, mog_dns, mog_rot, mog_scl, torch.zeros_like(mog_dns)], dim=1)
+            particle_material = torch.concat([mog_malb, mog_mrgh, mog_mmet], dim=1)
+            light_buffers = light_buffers.to(ray_dir.device)
+            lights, lat, mlv, mlt, ml, mlta = light_buffers.tensors()
+            (ray_radiance, ray_density, ray_hit_distance, ray_hit_distance_second_moment, ray_depth_distortion,
+             ray_normals, ray_shadingnormal, ray_material, hits_count, mog_visibility, ray_pbr, ray_light,
+             pbr_components) = tracer_wrapper.trace(frame_id, ray_to_world, ray_ori, ray_dir, particle_density,
+                                                    particle_material, mog_sph, mog_snrm, environment,
+                                                    environment_alias_table, lights, lat, mlv, mlt, ml, mlta,
+                                                    render_opts, sph_degree, min_transmittance, max_bounces)
+            ctx.save_for_backward(ray_to_world, ray_ori, ray_dir, ray_radiance, ray_density, ray_hit_distance,
+                                  ray_hit_distance_second_moment, ray_depth_distortion, ray_normals,
+                                  ray_shadingnormal, ray_material, ray_pbr, ray_light, particle_density,
+                                  particle_material, mog_sph, mog_snrm, environment, environment_alias_table, lights,
+                                  lat, mlv, mlt, ml, mlta)
+            ctx.frame_id = frame_id
+            ctx.render_opts = render_opts
+            ctx.sph_degree = sph_degree
+            ctx.min_transmittance = min_transmittance
+            ctx.max_bounces = max_bounces
+            ctx.tracer_wrapper = tracer_wrapper
+            return (ray_radiance, ray_density, ray_hit_distance[:, :, :, 0:1], ray_hit_distance_second_moment,
+                    ray_depth_distortion, ray_normals, ray_shadingnormal, ray_material, hits_count, mog_visibility,
+                    ray_pbr, ray_light, pbr_components)
+
+        @staticmethod
+        def backward(ctx, ray_radiance_grd, ray_density_grd, ray_hit_distance_grd, ray_hit_distance_second_moment_grd,
+                     ray_depth_distortion_grd, ray_normals_grd, ray_shadingnormal_grd, ray_material_grd,
+                     ray_hits_count_grd_UNUSED, mog_visibility_grd_UNUSED, ray_pbr_grd, ray_light_grd,
+                     pbr_components_grd_UNUSED):
+            (ray_to_world, ray_ori, ray_dir, ray_radiance, ray_density, ray_hit_distance,
+             ray_hit_distance_second_moment, ray_depth_distortion, ray_normals, ray_shadingnormal, ray_material,
+             ray_pbr, ray_light, particle_density, particle_material, mog_sph, mog_snrm, environment,
+             environment_alias_table, lights, lat, mlv, mlt, ml, mlta) = ctx.saved_tensors
+            if ray_light_grd is None:
+                ray_light_grd = torch.zeros_like(ray_light)
+            if ray_pbr_grd is None:
+                ray_pbr_grd = torch.zeros_like(ray_pbr)
+            if ray_material_grd is not None:
+                ray_material_grd = ray_material_grd.contiguous()
+            (particle_density_grd, particle_material_grd, mog_sph_grd, mog_sn_grd,
+             environment_grd) = ctx.tracer_wrapper.trace_bwd(
+                ctx.frame_id, ray_to_world, ray_ori, ray_dir, ray_radiance, ray_density, ray_hit_distance,
+                ray_hit_distance_second_moment, ray_depth_distortion, ray_normals, ray_shadingnormal, ray_material,
+                ray_pbr, ray_light, particle_density, particle_material, mog_sph, mog_snrm, environment,
+                environment_alias_table, lights, lat, mlv, mlt, ml, mlta, ray_radiance_grd, ray_density_grd,
+                ray_hit_distance_grd, ray_hit_distance_second_moment_grd, ray_depth_distortion_grd, ray_normals_grd,
+                ray_shadingnormal_grd, ray_material_grd, ray_pbr_grd.contiguous(), ray_light_grd.contiguous(),
+                ctx.render_opts, ctx.sph_degree, ctx.min_transmittance, ctx.max_bounces)
+            mog_pos_grd, mog_dns_grd, mog_rot_grd, mog_scl_grd, _ = torch.split(particle_density_grd, [3, 1, 4, 3, 1],
+                                                                                dim=1)
+            mog_malb_grd, mog_mrgh_grd, mog_mmet_grd = torch.split(particle_material_grd, [3, 1, 1], dim=1)
+            return (None, None, None, None, None, mog_pos_grd, mog_rot_grd, mog_scl_grd, mog_dns_grd, mog_sph_grd,
+                    mog_sn_grd, mog_malb_grd, mog_mrgh_grd, mog_mmet_grd, environment_grd, None, None, None, None,
+                    None, None)
+
+    class RenderOpts(IntEnum):
+        NONE = 0
+        INDIRECT = 1
+        DEFAULT = NONE
+
+    _MULTISPP_PATTERNS = {
+        1: ((0.5000, 0.5000),),
+        2: ((0.2500, 0.2500), (0.7500, 0.7500)),
+        4: ((0.3750, 0.1250), (0.8750, 0.3750), (0.6250, 0.8750), (0.1250, 0.6250)),
+        8: ((0.5625, 0.6875), (0.4375, 0.3125), (0.8125, 0.4375), (0.3125, 0.8125), (0.1875, 0.1875),
+            (0.0625, 0.5625), (0.6875, 0.0625), (0.9375, 0.9375)),
+        16: ((0.5625, 0.4375), (0.4375, 0.6875), (0.3125, 0.3750), (0.7500, 0.5625), (0.1875, 0.6250),
+             (0.6250, 0.1875), (0.1875, 0.3125), (0.6875, 0.8125), (0.3750, 0.1250), (0.5000, 0.9375),
+             (0.2500, 0.8750), (0.1250, 0.2500), (0.0000, 0.5000), (0.9375, 0.7500), (0.8750, 0.0625),
+             (0.0625, 0.0000)),
+        32: tuple(((x + 0.5) / 8.0, (y + 0.5) / 4.0) for y in range(4) for x in range(8)),
+    }
+
+    def __init__(self, conf):
+        self.device = "cuda"
+        self.conf = conf
+        r = _conf_get(conf, "render")
+        self.render_conf = r
+        self.num_update_bvh = 0
+        self._warned_spp_fallback = False
+        self.pred_pbr_filter = Filter(_conf_get(r, "filter_type", "none"))
+        self.visualize_lights = bool(_conf_get(r, "visualize_lights", False))
+        model = _conf_get(conf, "model")
+        enable_metallic = _conf_get(r, "enable_metallic", None)
+        if enable_metallic is None:
+            enable_metallic = _conf_get(model, "optimize_material_metallic", False)
+        self.tracer_wrapper = HipTracer(
+            "", "", _conf_get(r, "pipeline_type", "reference"),
+            _conf_get(r, "backward_pipeline_type", "referenceBwd"), _conf_get(r, "primitive_type", "instances"),
+            _conf_get(r, "particle_kernel_degree", 4), _conf_get(r, "particle_kernel_min_response", 0.0113),
+            _conf_get(r, "particle_kernel_density_clamping", True), _conf_get(r, "particle_radiance_sph_degree", 3),
+            _conf_get(r, "enable_normals", True), _conf_get(r, "enable_hitcounts", True),
+            _conf_get(r, "enable_mis", True), bool(enable_metallic), self.visualize_lights,
+            kbuffer=int(_conf_get(r, "kbuffer_depth", 16)))
+        self.timings = {}
+
+    # -------------------------------------------------------------- helpers (tracer.py:505-714)
+    def _postprocess_pbr_outputs(self, outputs):
+        pred_opacity = outputs.get("pred_opacity")
+        keys = ("pred_pbr", "pred_direct", "pred_indirect")
+        if pred_opacity is not None:
+            for key in keys:
+                if outputs.get(key) is not None:
+                    outputs[key] = outputs[key] * pred_opacity
+        for key in (*keys, "pred_light"):
+            if outputs.get(key) is not None:
+                outputs[key] = self.pred_pbr_filter(outputs[key])
+
+    @staticmethod
+    def _normalize_depth_output(outputs):
+        d, o = outputs.get("pred_dist"), outputs.get("pred_opacity")
+        if d is not None and o is not None:
+            outputs["pred_dist"] = torch.nan_to_num(d / o, 0.0, 0.0)
+
+    def _get_spp(self, train: bool) -> int:
+        key = "inversion_spp" if train else "render_spp"
+        return max(1, int(_conf_get(self.render_conf, key, 1)))
+
+    def _get_spp_chunk(self, spp: int) -> int:
+        return min(spp, max(1, int(_conf_get(self.render_conf, "spp_chunk", spp))))
+
+    @staticmethod
+    def _intrinsics(gpu_batch, dtype, device):
+        intr = getattr(gpu_batch, "intrinsics", None)
+        if intr is not None:
+            t = torch.as_tensor(intr, dtype=dtype, device=device).flatten()
+            if t.numel() == 4:
+                return tuple(t)
+            if t.numel() == 9:
+                K = t.reshape(3, 3)
+                return K[0, 0], K[1, 1], K[0, 2], K[1, 2]
+            return None
+        params = getattr(gpu_batch, "intrinsics_OpenCVPinholeCameraModelParameters", None)
+        if params is None:
+            return None
+
+        def get(name):
+            return params.get(name) if isinstance(params, dict) else getattr(params, name, None)
+
+        for name in ("radial_coeffs", "tangential_coeffs", "thin_prism_coeffs"):
+            c = get(name)
+            if c is not None:
+                c = torch.as_tensor(c, dtype=dtype, device=device).flatten()
+                if c.numel() > 0 and bool(torch.any(torch.abs(c) > 1e-8)):
+                    return None
+        focal, principal = get("focal_length"), get("principal_point")
+        if focal is None or principal is None:
+            return None
+        focal = torch.as_tensor(focal, dtype=dtype, device=device).flatten()
+        principal = torch.as_tensor(principal, dtype=dtype, device=device).flatten()
+        fx = focal[0]
+        fy = focal[1] if focal.numel() > 1 else focal[0]
+        return fx, fy, principal[0], principal[1]
+
+    def _make_spp_jitter(self, spp, h, w, device, dtype, frame_id):
+        if spp in self._MULTISPP_PATTERNS:
+            return torch.tensor(self._MULTISPP_PATTERNS[spp], dtype=dtype, device=device).view(spp, 1, 1, 2)
+        grid_w = int(math.ceil(math.sqrt(spp)))
+        grid_h = int(math.ceil(spp / grid_w))
+        idx = torch.arange(spp, dtype=dtype, device=device)
+        jitter = torch.stack(((torch.remainder(idx, grid_w) + 0.5) / grid_w,
+                              (torch.div(idx, grid_w, rounding_mode="floor") + 0.5) / grid_h), dim=-1)
+        g = torch.Generator(device=device)
+        g.manual_seed((int(frame_id) + 1) * 1315423911 + spp * 2654435761)
+        shift = torch.rand((1, 2), dtype=dtype, device=device, generator=g)
+        return torch.remainder(jitter + shift, 1.0).view(spp, 1, 1, 2)
+
+    def _expand_rays_for_spp(self, gpu_batch, spp, frame_id, jitter=None):
+        rays_ori = gpu_batch.rays_ori.contiguous()
+        rays_dir = gpu_batch.rays_dir.contiguous()
+        if spp == 1 and jitter is None:
+            return rays_ori, rays_dir, 1
+        pixel_coords = getattr(gpu_batch, "pixel_coords", None)
+        intr = self._intrinsics(gpu_batch, rays_dir.dtype, rays_dir.device)
+        if intr is None or pixel_coords is None:
+            return rays_ori, rays_dir, 1
+        base, h, w, _ = rays_dir.shape
+        fx, fy, cx, cy = intr
+        po = pixel_coords.to(dtype=rays_dir.dtype, device=rays_dir.device) - 0.5
+        if jitter is None:
+            jitter = self._make_spp_jitter(spp, h, w, rays_dir.device, rays_dir.dtype, frame_id)
+        po = po.unsqueeze(0)
+        jitter = jitter.view(spp, 1, 1, 1, 2) if jitter.shape[1:3] == (1, 1) else jitter.view(spp, 1, h, w, 2)
+        dirs = torch.stack(((po[..., 0] + jitter[..., 0] - cx) / fx, (po[..., 1] + jitter[..., 1] - cy) / fy,
+                            torch.ones((spp, base, h, w), dtype=rays_dir.dtype, device=rays_dir.device)), dim=-1)
+        dirs = torch.nn.functional.normalize(dirs, dim=-1).reshape(spp * base, h, w, 3).contiguous()
+        origins = rays_ori.unsqueeze(0).expand(spp, *rays_ori.shape).reshape(spp * base, h, w, 3).contiguous()
+        return origins, dirs, spp
+
+    @staticmethod
+    def _average_spp_output(value, spp, base_batch):
+        if spp == 1:
+            return value
+        return value.reshape(spp, base_batch, *value.shape[1:]).mean(dim=0)
+
+    # -------------------------------------------------------------- build_acc (tracer.py:716-738)
+    def build_acc(self, gaussians, rebuild=True):
+        r = self.render_conf
+        max_upd = int(_conf_get(r, "max_consecutive_bvh_update", 15))
+        clamping = bool(_conf_get(r, "particle_kernel_density_clamping", True))
+        allow_update = max_upd > 1 and not clamping
+        rebuild_bvh = rebuild or clamping or self.num_update_bvh >= max_upd
+        self.tracer_wrapper.build_bvh(
+            gaussians.positions.view(-1, 3).contiguous(),
+            gaussians.rotation_activation(gaussians.rotation).view(-1, 4).contiguous(),
+            gaussians.scale_activation(gaussians.scale).view(-1, 3).contiguous(),
+            gaussians.density_activation(gaussians.density).view(-1, 1).contiguous(),
+            rebuild_bvh, allow_update)
+        self.num_update_bvh = 0 if rebuild_bvh else self.num_update_bvh + 1
+
+    # -------------------------------------------------------------- render (tracer.py:740-975)
+    def render(self, gaussians, gpu_batch, train=False, frame_id=0, sh_indirect: bool = False):
+        base_batch = gpu_batch.rays_dir.shape[0]
+        spp = self._get_spp(train)
+        if spp > 1 and (getattr(gpu_batch, "pixel_coords", None) is None or
+                        self._intrinsics(gpu_batch, gpu_batch.rays_dir.dtype, gpu_batch.rays_dir.device) is None):
+            if not self._warned_spp_fallback:
+                logger.warning("PTIR SPP fallback to spp=1: batch has no supported pinhole intrinsics/pixel_coords")
+                self._warned_spp_fallback = True
+            spp = 1
+        spp_chunk = self._get_spp_chunk(spp)
+        h, w = gpu_batch.rays_dir.shape[1:3]
+        spp_jitter = None
+        if spp > 1:
+            spp_jitter = self._make_spp_jitter(spp, h, w, gpu_batch.rays_dir.device, gpu_batch.rays_dir.dtype,
+                                               frame_id)
+        acc = {}
+        mog_visibility = None
+        total_spp = 0
+        for spp_start in range(0, spp, spp_chunk):
+            chunk_spp = min(spp_chunk, spp - spp_start)
+            chunk_jitter = None if spp_jitter is None else spp_jitter[spp_start:spp_start + chunk_spp]
+            rays_ori, rays_dir, chunk_spp = self._expand_rays_for_spp(gpu_batch, chunk_spp, frame_id + spp_start,
+                                                                      jitter=chunk_jitter)
+            environment = gaussians.get_environment()
+            if environment is None:
+                environment = torch.empty((0, 0, 4), dtype=torch.float32, device=rays_dir.device)
+            table = getattr(gaussians, "environment_alias_table", None)
+            if table is None:
+                env_table = torch.empty(0, dtype=torch.float32, device=rays_dir.device)
+            else:
+                env_table = torch.concat([
+                    table.prob.reshape(1, table.height, table.width),
+                    table.alias.reshape(1, table.height, table.width).to(dtype=torch.float32),
+                    table.pdf.reshape(1, table.height, table.width)], dim=0).to(device=rays_dir.device).contiguous()
+            light_buffers = LightBuffers.from_batch(gpu_batch, device=rays_dir.device)
+            outs = Tracer._Autograd.apply(
+                self.tracer_wrapper, frame_id + spp_start, gpu_batch.T_to_world.contiguous(), rays_ori, rays_dir,
+                gaussians.positions.contiguous(), gaussians.get_rotation().contiguous(),
+                gaussians.get_scale().contiguous(), gaussians.get_density().contiguous(),
+                gaussians.get_features().contiguous(), gaussians.get_shading_normal().contiguous(),
+                gaussians.get_material_albedo().contiguous(), gaussians.get_material_roughness().contiguous(),
+                gaussians.get_material_metallic().contiguous(), environment, env_table, light_buffers,
+                int(Tracer.RenderOpts.INDIRECT if sh_indirect else Tracer.RenderOpts.DEFAULT),
+                gaussians.n_active_features, _conf_get(self.render_conf, "min_transmittance", 1e-3),
+                _conf_get(self.render_conf, "max_bounces", 3))
+            (c_rgb, c_opacity, c_dist, c_dist2, c_distortion, c_normals, c_snormal, c_material, c_hits, c_vis, c_pbr,
+             c_light, c_comp) = outs
+            background = getattr(gaussians, "background", None)
+            if callable(background):
+                c_rgb, c_opacity = background(gpu_batch.T_to_world.contiguous(), rays_dir, c_rgb, c_opacity, train)
+            for name, value in (("pred_rgb", c_rgb), ("pred_opacity", c_opacity), ("pred_dist", c_dist),
+                                ("pred_depth_second_moment", c_dist2), ("pred_depth_distortion", c_distortion),
+                                ("pred_normals", c_normals), ("pred_shadingnormal", c_snormal),
+                                ("pred_material", c_material), ("hits_count", c_hits), ("pred_pbr", c_pbr),
+                                ("pred_light", c_light), ("pbr_components", c_comp.detach())):
+                avg = self._average_spp_output(value, chunk_spp, base_batch) * chunk_spp
+                acc[name] = avg if name not in acc else acc[name] + avg
+            mog_visibility = c_vis if mog_visibility is None else torch.maximum(mog_visibility, c_vis)
+            total_spp += chunk_spp
+        acc = {k: v / total_spp for k, v in acc.items()}
+        comp = acc.pop("pbr_components", None)
+        if comp is not None:
+            acc["pred_direct"] = comp[..., 0, :]
+            acc["pred_indirect"] = comp[..., 1, :]
+        self._normalize_depth_output(acc)
+        self._postprocess_pbr_outputs(acc)
+        keys = ("pred_rgb", "pred_opacity", "pred_dist", "pred_depth_second_moment", "pred_depth_distortion",
+                "pred_normals", "pred_shadingnormal", "pred_material", "pred_pbr", "pred_light", "pred_direct",
+                "pred_indirect", "hits_count")
+        out = {k: acc.get(k) for k in keys}
+        if out["pred_normals"] is not None:
+            out["pred_normals"] = torch.nn.functional.normalize(out["pred_normals"], dim=3)
+        out = {k: v for k, v in out.items() if v is not None}
+        out["frame_time_ms"] = 0.0
+        out["mog_visibility"] = mog_visibility
+        return post_processing(out, gpu_batch, self.visualize_lights)

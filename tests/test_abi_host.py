@@ -1,0 +1,152 @@
+"""CPU tests of the C ABI library (load, exports, argument checking) and the host-side
+Python surface.  Nothing here launches a kernel."""
+
+import ctypes as C
+import os
+import re
+
+import pytest
+import torch
+
+from ptir_amd import _lib, envmap, scenes
+from ptir_amd.filters import Filter, apply_gt_mask_to_tensor, post_processing
+from ptir_amd.tracer import HipTracer, LightBuffers, Tracer
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_symbols():
+    text = open(os.path.join(ROOT, "include", "ptir_amd.h")).read()
+    return sorted(set(re.findall(r"^(?:ptir_status|const char\*)\s+(ptir_\w+)\s*\(", text, re.M)))
+
+
+def test_header_declares_the_abi():
+    assert _declared_symbols() == sorted(_lib.EXPORTED)
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    L = _lib.lib()
+    for name in _declared_symbols():
+        assert hasattr(L, name), name
+    assert b"gfx950" in L.ptir_version()
+
+
+def _opts(**kw):
+    o = dict(kernel_degree=4.0, min_response=0.0113, density_clamping=1, sph_max_degree=3, enable_normals=1,
+             enable_hitcounts=1, enable_mis=1, enable_metallic=0, visualize_lights=0, kbuffer=16)
+    o.update(kw)
+    return _lib.Options(**o)
+
+
+def test_create_destroy_and_option_errors():
+    L = _lib.lib()
+    h = C.c_void_p()
+    assert L.ptir_create(C.byref(_opts()), C.byref(h)) == 0
+    assert h.value
+    assert L.ptir_destroy(h) == 0
+    for bad, status in [(dict(kernel_degree=6.0), 2), (dict(kbuffer=12), 1), (dict(visualize_lights=1), 2),
+                        (dict(min_response=0.0), 1), (dict(sph_max_degree=4), 2)]:
+        h = C.c_void_p()
+        assert L.ptir_create(C.byref(_opts(**bad)), C.byref(h)) == status
+        assert not h.value
+        assert L.ptir_last_error(None)
+
+
+def test_trace_before_build_and_refit_without_build_are_errors():
+    L = _lib.lib()
+    h = C.c_void_p()
+    assert L.ptir_create(C.byref(_opts()), C.byref(h)) == 0
+    sc = _lib.Scene()
+    la = _lib.Launch()
+    out = _lib.FwdOutputs()
+    assert L.ptir_trace(h, C.byref(sc), C.byref(la), C.byref(out), None, None) == 3  # NOT_BUILT
+    assert b"before build_bvh" in L.ptir_last_error(h)
+    # refit (rebuild = 0) of a hierarchy that was never built is rejected before any GPU work
+    assert L.ptir_build_bvh(h, None, None, None, None, 10, 0, 0, None) == 1
+    assert L.ptir_build_bvh(h, None, None, None, None, 10, 1, 0, None) == 1  # null tensors
+    # an empty scene builds without touching the GPU
+    assert L.ptir_build_bvh(h, None, None, None, None, 0, 1, 0, None) == 0
+    sc.num_lights = 1
+    assert L.ptir_trace(h, C.byref(sc), C.byref(la), C.byref(out), None, None) == 2  # lights unsupported
+    L.ptir_destroy(h)
+
+
+def test_hip_tracer_rejects_unsupported_pipelines_and_host_tensors():
+    with pytest.raises(ValueError):
+        HipTracer(pipeline="referenceSlang")
+    with pytest.raises(ValueError):
+        HipTracer(primitive="icosahedron")
+    with pytest.raises(_lib.PtirError):
+        HipTracer(particle_kernel_degree=7)
+    t = HipTracer()
+    with pytest.raises(ValueError):
+        t.build_bvh(torch.zeros(4, 3), torch.zeros(4, 4), torch.zeros(4, 3), torch.zeros(4, 1))
+
+
+def test_light_buffers_empty_shapes():
+    lb = LightBuffers.empty("cpu")
+    shapes = [tuple(x.shape) for x in lb.tensors()]
+    assert shapes == [(0, 9), (5, 0), (0, 3), (0, 3), (0, 8), (3, 0)]
+    assert lb.mesh_triangles.dtype == torch.int32
+
+
+def test_atrous_filter_matches_direct_convolution():
+    x = torch.rand(2, 9, 7, 3, generator=torch.Generator().manual_seed(0))
+    y = Filter("atrous")(x)
+    k = torch.tensor([[1, 2, 1], [2, 4, 2], [1, 2, 1]], dtype=torch.float32) / 16
+    xp = torch.nn.functional.pad(x.permute(0, 3, 1, 2), (1, 1, 1, 1), mode="replicate").permute(0, 2, 3, 1)
+    ref = torch.zeros_like(x)
+    for dy in range(3):
+        for dx in range(3):
+            ref += k[dy, dx] * xp[:, dy:dy + 9, dx:dx + 7]
+    torch.testing.assert_close(y, ref, rtol=1e-6, atol=1e-6)
+    assert torch.equal(Filter("none")(x), x)
+    with pytest.raises(ValueError):
+        Filter("gaussian")
+
+
+def test_gt_mask_only_touches_material_light_and_shading_normal():
+    class B:
+        mask = torch.zeros(1, 4, 4, 1)
+
+    outs = {k: torch.ones(1, 4, 4, 3) for k in ("pred_pbr", "pred_light", "pred_material", "pred_shadingnormal")}
+    res = post_processing(outs, B())
+    assert torch.equal(res["pred_pbr"], outs["pred_pbr"])
+    for k in ("pred_light", "pred_material", "pred_shadingnormal"):
+        assert float(res[k].abs().sum()) == 0
+    assert res["_pbr_gt_mask_applied"]
+    assert apply_gt_mask_to_tensor(torch.ones(1, 4, 4, 3), torch.ones(4, 4)).shape == (1, 4, 4, 3)
+
+
+def test_spp_expansion_follows_the_reference_patterns():
+    class Batch:
+        pass
+
+    h, w = 6, 5
+    b = Batch()
+    b.rays_dir = torch.zeros(1, h, w, 3)
+    b.rays_ori = torch.zeros(1, h, w, 3)
+    ys, xs = torch.meshgrid(torch.arange(h) + 0.5, torch.arange(w) + 0.5, indexing="ij")
+    b.pixel_coords = torch.stack([xs, ys], -1).unsqueeze(0)
+    b.intrinsics = [10.0, 10.0, w / 2, h / 2]
+    tr = Tracer.__new__(Tracer)
+    o, d, spp = tr._expand_rays_for_spp(b, 4, 0)
+    assert spp == 4 and d.shape == (4, h, w, 3)
+    torch.testing.assert_close(d.norm(dim=-1), torch.ones(4, h, w))
+    # sample 0 of the 4-spp pattern sits at (0.375, 0.125) inside the pixel
+    px = torch.tensor([2 + 0.375 - w / 2, 3 + 0.125 - h / 2, 10.0]) / 10.0
+    torch.testing.assert_close(d[0, 3, 2], px / px.norm())
+    # the scenes module uses the same pattern table
+    for k, v in scenes.MULTISPP_PATTERNS.items():
+        assert Tracer._MULTISPP_PATTERNS[k] == v
+
+
+def test_env_table_packing_layout():
+    env = scenes.lognormal_env(16, 32, torch.Generator().manual_seed(1))
+    t = envmap.build_environment_alias_table(env, target_size=(8, 16))
+    packed = envmap.pack_alias_table(t)
+    assert packed.shape == (3, 8, 16)
+    torch.testing.assert_close(packed[0].reshape(-1), t.prob)
+    torch.testing.assert_close(packed[1].reshape(-1), t.alias.float())
+    torch.testing.assert_close(packed[2].reshape(-1), t.pdf)
+    assert envmap.pack_alias_table(None).numel() == 0
