@@ -61,7 +61,7 @@ __device__ __forceinline__ bool slab(float4 lo, float4 hi, const Ray& r, V3 inv,
 // Collect the K closest candidates in (tmin, tmax).  Returns the count; the sorted
 // entries are left in LDS: t at words [0,K), leaf slot at words [K,2K).
 template <int K>
-__device__ int collect(const DeviceScene& S, const Ray& r, float tmin, float tmax, const Lane& L) {
+__device__ __forceinline__ int collect(const DeviceScene& S, const Ray& r, float tmin, float tmax, const Lane& L) {
     float bt[K];
     uint32_t bs[K];
 #pragma unroll
@@ -156,7 +156,7 @@ struct Seg {  // payLoad.h:38-100 rayPayload
     float scatterPdf, lightPdf;
     V3 dirGrad;
     bool hit, valid;
-    __device__ void reset(const Ray& r, float last) {
+    __device__ __forceinline__ void reset(const Ray& r, float last) {
         ray = r;
         ivalid = false;
         pos = sn = alb = galb = v3s(0.f);
@@ -261,7 +261,7 @@ __device__ __forceinline__ V3 background(const DeviceScene& S, V3 rd) {
 }
 
 // getBackgroundColorBwd + getBackgroundRayDirectionGradFast (environment.cuh:421-473)
-__device__ V3 background_bwd(const DeviceScene& S, V3 rd, V3 cg, V3& dirGrad, float* denv) {
+__device__ __forceinline__ V3 background_bwd(const DeviceScene& S, V3 rd, V3 cg, V3& dirGrad, float* denv) {
     if (!has_env(S)) return v3s(0.f);
     const V3 dir = env_rotate(rd);
     float u, v;
@@ -294,7 +294,7 @@ __device__ V3 background_bwd(const DeviceScene& S, V3 rd, V3 cg, V3& dirGrad, fl
 }
 
 // sampleEnvironmentAliasDirection (environment.cuh:227-255); draws u1..u4 in order
-__device__ V3 sample_env(const DeviceScene& S, Rng& rng, float& pdf) {
+__device__ __forceinline__ V3 sample_env(const DeviceScene& S, Rng& rng, float& pdf) {
     const int nc = S.tab_h * S.tab_w;
     const float* prob = S.tab;
     const float* alias = S.tab + nc;
@@ -318,7 +318,7 @@ __device__ V3 sample_env(const DeviceScene& S, Rng& rng, float& pdf) {
     return env_inverse_rotate(v3(st * cp, ct * cp, -sp));
 }
 
-__device__ float env_pdf(const DeviceScene& S, V3 rd) {
+__device__ __forceinline__ float env_pdf(const DeviceScene& S, V3 rd) {
     if (!has_env_light(S)) return 0.f;
     const V3 dir = env_rotate(rd);
     float u, v;
@@ -366,7 +366,7 @@ struct BG {
 };
 
 template <bool GRADS>
-__device__ BG eval_ls(V3 wo, V3 n, V3 albedo, float metal, float roughness, V3 ldir, float& pdf, bool metallic) {
+__device__ __forceinline__ BG eval_ls(V3 wo, V3 n, V3 albedo, float metal, float roughness, V3 ldir, float& pdf, bool metallic) {
     BG res;
     res.value = res.dAlb = res.dMet = res.dRough = res.dDir = v3s(0.f);
     const V3 L = fsn(ldir, n);
@@ -417,7 +417,7 @@ __device__ BG eval_ls(V3 wo, V3 n, V3 albedo, float metal, float roughness, V3 l
     return res;
 }
 
-__device__ float pdf_rough_grad(V3 wo, V3 n, float roughness, V3 ldir) {
+__device__ __forceinline__ float pdf_rough_grad(V3 wo, V3 n, float roughness, V3 ldir) {
     const V3 L = fsn(ldir, n);
     const float NV = sat(dot(n, wo)), NL = sat(dot(n, L));
     if (NV <= 0.f || NL <= 0.f) return 0.f;
@@ -434,7 +434,7 @@ __device__ float pdf_rough_grad(V3 wo, V3 n, float roughness, V3 ldir) {
 }
 
 template <bool GRADS>
-__device__ BG sample_throughput(V3 wo, V3 n, V3 albedo, float metal, float roughness, V3 rnd, V3& L, float& pdf,
+__device__ __forceinline__ BG sample_throughput(V3 wo, V3 n, V3 albedo, float metal, float roughness, V3 rnd, V3& L, float& pdf,
                                 bool metallic) {
     const float rough = clamp_rough(roughness);
     BG res;
@@ -524,7 +524,7 @@ __device__ __forceinline__ uint32_t orig_id(const DeviceScene& S, int slot) {
     return uint32_t(__float_as_int(S.grec[4 * slot + 3].z));
 }
 
-__device__ V3 sh_radiance(const DeviceScene& S, uint32_t id, V3 d) {
+__device__ __forceinline__ V3 sh_radiance(const DeviceScene& S, uint32_t id, V3 d) {
     const float* c = S.sh + size_t(S.sh_stride) * id;
     auto C = [&](int k) { return v3(c[3 * k], c[3 * k + 1], c[3 * k + 2]); };
     V3 r = 0.28209479177387814f * C(0);
@@ -553,8 +553,8 @@ __device__ V3 sh_radiance(const DeviceScene& S, uint32_t id, V3 d) {
 
 // pathTracer.cuh:216-311 rayIntersect with gaussianParticles.cuh:343-440 processHit
 template <int K>
-__device__ void ray_intersect(const DeviceScene& S, const Ray ray, Seg& pl, Rng& rng, bool selfOcc, bool primary,
-                              uint32_t* vis, const Lane& L) {
+__device__ __forceinline__ void ray_intersect(const DeviceScene& S, const Ray ray, Seg& pl, Rng& rng, bool selfOcc, bool primary,
+                              bool gbuffer, uint32_t* vis, const Lane& L) {
     float t0, t1;
     scene_interval(S, ray, t0, t1);
     const float sPdf = pl.scatterPdf, lPdf = pl.lightPdf;
@@ -565,8 +565,10 @@ __device__ void ray_intersect(const DeviceScene& S, const Ray ray, Seg& pl, Rng&
     V3 alb = v3s(0.f), snAcc = v3s(0.f);
     float rough = 0.f, metal = 0.f;
     const float minT = S.min_transmittance;
-    const bool need_rad = primary || S.render_opts == 1;
-    const bool need_nrm = primary && S.enable_normals;
+    // radiance and normals feed only the G-buffer outputs (and, with INDIRECT, the
+    // secondary radiance); the replay in the backward needs neither
+    const bool need_rad = (primary && gbuffer) || S.render_opts == 1;
+    const bool need_nrm = primary && gbuffer && S.enable_normals;
     const float dlen = sqrtf(dot(ray.d, ray.d));
     count(S, C_SEGMENTS);
     while (pl.lastHit <= t1 && pl.T > minT) {
@@ -634,7 +636,7 @@ __device__ void ray_intersect(const DeviceScene& S, const Ray ray, Seg& pl, Rng&
 
 // pathTracer.cuh:161-214 stochastic shadow ray
 template <int K>
-__device__ bool shadow_occluded(const DeviceScene& S, const Ray ray, Rng& rng, float tmin, float tmax, const Lane& L) {
+__device__ __forceinline__ bool shadow_occluded(const DeviceScene& S, const Ray ray, Rng& rng, float tmin, float tmax, const Lane& L) {
     float t0, t1;
     scene_interval(S, ray, t0, t1);
     float startT = fmaxf(tmin, t0 - kEpsT);
@@ -663,7 +665,7 @@ __device__ bool shadow_occluded(const DeviceScene& S, const Ray ray, Rng& rng, f
 
 // pathTracer.cuh:313-390 material-gradient flush over a replayed segment
 template <int K>
-__device__ void intersect_bwd(const DeviceScene& S, const Ray ray, float opacity, float maxHit, V3 ga, float gr,
+__device__ __forceinline__ void intersect_bwd(const DeviceScene& S, const Ray ray, float opacity, float maxHit, V3 ga, float gr,
                               float gm, bool selfOcc, float* dmat_leaf, const Lane& L) {
     const float inv = 1.f / fmaxf(opacity, 1e-12f);
     ga = ga * inv;
@@ -706,7 +708,7 @@ __device__ void intersect_bwd(const DeviceScene& S, const Ray ray, float opacity
 
 // ---------------------------------------------------------------- path logic
 template <int K>
-__device__ void sample_nee(const DeviceScene& S, Path& p, Rng& rng, const Lane& L) {
+__device__ __forceinline__ void sample_nee(const DeviceScene& S, Path& p, Rng& rng, const Lane& L) {
     emit_clear(p);
     if (!has_env_light(S)) return;
     float envPdf;
@@ -727,7 +729,7 @@ __device__ void sample_nee(const DeviceScene& S, Path& p, Rng& rng, const Lane& 
     p.econtrib = p.thr * bc.value * Li / fmaxf(envPdf, 1e-6f);
 }
 
-__device__ void accumulate(const DeviceScene& S, Path& p) {
+__device__ __forceinline__ void accumulate(const DeviceScene& S, Path& p) {
     p.cur.contribution = v3s(0.f);
     const bool first = p.numBounces == 1u;
     const bool mis_on = S.enable_mis != 0;
@@ -773,7 +775,7 @@ __device__ void accumulate(const DeviceScene& S, Path& p) {
     }
 }
 
-__device__ void accumulate_bwd(const DeviceScene& S, Path& p, float* denv) {
+__device__ __forceinline__ void accumulate_bwd(const DeviceScene& S, Path& p, float* denv) {
     p.cur.contribution = v3s(0.f);
     const bool first = p.numBounces == 1u;
     const bool mis_on = S.enable_mis != 0;
@@ -813,7 +815,7 @@ __device__ void accumulate_bwd(const DeviceScene& S, Path& p, float* denv) {
     }
 }
 
-__device__ void sample_next(const DeviceScene& S, Path& p, Rng& rng) {
+__device__ __forceinline__ void sample_next(const DeviceScene& S, Path& p, Rng& rng) {
     V3 n, wo, alb;
     float met;
     shading_frame(S, p.cur, n, wo, alb, met);
@@ -841,7 +843,7 @@ __device__ void sample_next(const DeviceScene& S, Path& p, Rng& rng) {
 }
 
 template <int K>
-__device__ void pending_bwd(const DeviceScene& S, Path& p, float* dmat, const Lane& L) {
+__device__ __forceinline__ void pending_bwd(const DeviceScene& S, Path& p, float* dmat, const Lane& L) {
     if (!p.pvalid) return;
     const float dR = dot(p.cur.dirGrad, p.pdir);
     p.pvalid = false;
@@ -849,7 +851,7 @@ __device__ void pending_bwd(const DeviceScene& S, Path& p, float* dmat, const La
     intersect_bwd<K>(S, p.pray, p.popacity, p.pmaxHit, v3s(0.f), dR, 0.f, p.pbounces > 1u, dmat, L);
 }
 
-__device__ void nee_grad_bwd(const DeviceScene& S, Path& p, const Seg& cs) {
+__device__ __forceinline__ void nee_grad_bwd(const DeviceScene& S, Path& p, const Seg& cs) {
     if (p.numBounces != 1u) return;
     if (p.elightPdf <= 0.f) return;
     V3 n, wo, alb;
@@ -865,7 +867,7 @@ __device__ void nee_grad_bwd(const DeviceScene& S, Path& p, const Seg& cs) {
 }
 
 template <int K>
-__device__ void sample_next_bwd(const DeviceScene& S, Path& p, Rng& rng, float* dmat, const Lane& L) {
+__device__ __forceinline__ void sample_next_bwd(const DeviceScene& S, Path& p, Rng& rng, float* dmat, const Lane& L) {
     V3 n, wo, alb;
     float met;
     shading_frame(S, p.cur, n, wo, alb, met);
@@ -978,7 +980,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_fwd(DeviceScene S, FwdOut
     Path p;
     path_init(p);
     count(S, C_PATHS);
-    ray_intersect<K>(S, primary_ray(S, pix), p.cur, rng, false, true, out.vis, L);
+    ray_intersect<K>(S, primary_ray(S, pix), p.cur, rng, false, true, true, out.vis, L);
     {
         const Seg& c = p.cur;
         float* o;
@@ -1001,7 +1003,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_fwd(DeviceScene S, FwdOut
             p.numBounces = depth + 1u;
             sample_next(S, p, rng);
             if (vmax(p.thr) < 1e-4f) break;
-            ray_intersect<K>(S, p.cur.ray, p.cur, rng, true, false, out.vis, L);
+            ray_intersect<K>(S, p.cur.ray, p.cur, rng, true, false, true, out.vis, L);
         }
     }
     float* o;
@@ -1029,7 +1031,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_bwd(DeviceScene S, BwdIO 
     p.dL = v3(io.pbr_grad[3 * pix], io.pbr_grad[3 * pix + 1], io.pbr_grad[3 * pix + 2]);
     p.Lnb = v3(io.light[3 * pix], io.light[3 * pix + 1], io.light[3 * pix + 2]);
     if (io.light_grad) p.dLnb = v3(io.light_grad[3 * pix], io.light_grad[3 * pix + 1], io.light_grad[3 * pix + 2]);
-    ray_intersect<K>(S, primary_ray(S, pix), p.cur, rng, false, true, nullptr, L);
+    ray_intersect<K>(S, primary_ray(S, pix), p.cur, rng, false, true, false, nullptr, L);
     if (!p.cur.ivalid) return;
     if (io.mat_grad) {
         const float* mg = io.mat_grad + 5 * pix;
@@ -1046,7 +1048,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_bwd(DeviceScene S, BwdIO 
         p.numBounces = depth + 1u;
         sample_next_bwd<K>(S, p, rng, io.dmat_leaf, L);
         if (vmax(p.thr) < 1e-4f) break;
-        ray_intersect<K>(S, p.cur.ray, p.cur, rng, true, false, nullptr, L);
+        ray_intersect<K>(S, p.cur.ray, p.cur, rng, true, false, false, nullptr, L);
     }
 }
 

@@ -74,11 +74,18 @@ class HipTracer:
         self.sph_max_degree = int(particle_radiance_sph_degree)
         self.last_counters: dict | None = None
 
-    def __del__(self):
+    def close(self) -> None:
+        """Free the device buffers of this tracer (also done on garbage collection)."""
         h = getattr(self, "_h", None)
+        self._h = None
         if h is not None and h.value:
-            _lib.lib().ptir_destroy(h)
-            self._h = None
+            try:
+                _lib.lib().ptir_destroy(h)
+            except Exception:  # interpreter shutdown: the library may already be unloaded
+                pass
+
+    def __del__(self):
+        self.close()
 
     # -------------------------------------------------------------- build_bvh
     def build_bvh(self, mog_pos, mog_rot, mog_scl, mog_dns, rebuild=True, allow_update=False):

@@ -1,0 +1,244 @@
+"""GPU parity of the HIP path tracer against the FP64 CPU oracle (and structural
+properties at full size).
+
+Tolerances.  The GPU computes in FP32, the oracle in FP64, both with the same PCG32
+streams.  Pixels whose path takes a different discrete decision (a Bernoulli shadow test,
+an escape test or a candidate at the acceptance threshold) can differ by O(1); all others
+agree to FP32 rounding.  So each output is checked as
+  * the fraction of pixels agreeing within rtol 1e-3 (atol 1e-5 absolute), and
+  * the relative L2 error of gradients, whose bar is 1e-3 of the oracle's norm.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from gpu_helpers import (env_table, gpu_backward, gpu_forward, oracle_forward, pixel_agreement, rel_l2, to_np)
+from oracle import oracle as O
+from ptir_amd import scenes
+from ptir_amd.tracer import HipTracer
+
+pytestmark = pytest.mark.gpu
+
+RTOL, ATOL = 1e-3, 1e-5
+
+
+@pytest.fixture(scope="module")
+def small():
+    sc = scenes.small_scene(0)
+    return sc, env_table(sc)
+
+
+@pytest.fixture(scope="module")
+def small_run(small, device):
+    sc, tab = small
+    tr = HipTracer(enable_metallic=True)
+    outs, args = gpu_forward(tr, sc, tab, device, counters=True)
+    torch.cuda.synchronize()
+    inp, rays, ref = oracle_forward(sc, tab)
+    return tr, outs, args, to_np(outs), inp, rays, ref
+
+
+def test_forward_gbuffer_parity(small_run):
+    _, _, _, g, _, _, ref = small_run
+    print({k: pixel_agreement(g[k], ref[k], RTOL, ATOL) for k in ("rad", "dns", "hit", "mat", "pbr")}, flush=True)
+    for k in ("rad", "dns", "hit", "hit2", "distortion", "nrm", "snrm", "mat", "hits"):
+        assert pixel_agreement(g[k], ref[k], RTOL, ATOL) > 0.995, k
+
+
+def test_forward_pbr_parity(small_run):
+    _, _, _, g, _, _, ref = small_run
+    for k in ("pbr", "light", "comp"):
+        assert pixel_agreement(g[k], ref[k], RTOL, ATOL) > 0.98, k
+        m_g, m_r = g[k].mean(), ref[k].mean()
+        assert abs(m_g - m_r) <= 1e-2 * abs(m_r), k
+
+
+def test_visibility_matches(small_run):
+    _, _, _, g, _, _, ref = small_run
+    diff = np.abs(g["visibility"] - ref["visibility"]).sum()
+    assert diff <= 1e-3 * ref["visibility"].sum() + 2
+
+
+def test_work_counters_match_oracle(small_run):
+    tr, _, _, _, _, _, ref = small_run
+    gc, oc = tr.last_counters, ref["counters"]
+    for k in ("paths", "segments", "candidates", "accepted", "accepted_primary", "shadow_candidates", "env_lookups"):
+        assert abs(gc[k] - oc[k]) <= 2e-3 * oc[k] + 2, (k, gc[k], oc[k])
+
+
+def test_backward_parity_ptir_mode(small, small_run, device):
+    sc, tab = small
+    tr, outs, args, _, inp, rays, _ = small_run
+    tgt = scenes.target_image(sc).to(device)
+    pbr = outs[10]
+    g = (2 * (pbr - tgt) / pbr.numel()).contiguous()
+    mg = torch.zeros_like(outs[7])
+    mg[..., 3] = 1e-4  # a roughness-output gradient exercises rayMaterialGrad
+    lg = torch.zeros_like(pbr)
+    lg[..., 1] = 1e-5  # and rayLightGrad
+    grads = gpu_backward(tr, sc, outs, args, g, lg, mg, counters=True)
+    torch.cuda.synchronize()
+    dmat, denv, cnt = O.trace_bwd(inp, rays, outs[10].double().cpu().numpy(), outs[11].double().cpu().numpy(),
+                                  g.cpu().numpy(), lg.cpu().numpy(), mg.cpu().numpy(), counters=True)
+    e_mat = rel_l2(grads[1].cpu().numpy(), dmat)
+    e_env = rel_l2(grads[4].cpu().numpy(), denv)
+    print(f"backward rel L2: material {e_mat:.3e} env {e_env:.3e}", flush=True)
+    assert e_mat < 1e-3
+    assert e_env < 1e-3
+    # reference semantics: only material and env gradients are produced
+    for t in (grads[0], grads[2], grads[3]):
+        assert float(t.abs().max()) == 0.0
+    gc = tr.last_counters
+    for k in ("mat_atomics", "env_atomics"):
+        assert abs(gc[k] - cnt[k]) <= 2e-3 * cnt[k] + 2, (k, gc[k], cnt[k])
+
+
+def test_forward_is_deterministic_and_kbuffer_invariant(small, device):
+    sc, tab = small
+    ref = None
+    for k in (8, 16, 32):
+        tr = HipTracer(enable_metallic=True, kbuffer=k)
+        for _ in range(2):
+            outs, _ = gpu_forward(tr, sc, tab, device)
+            cur = [o.clone() for o in outs]
+            if ref is None:
+                ref = cur
+            else:
+                for a, b in zip(ref, cur):
+                    assert torch.equal(a, b)
+
+
+def test_backward_is_linear_in_the_output_gradient(small, small_run, device):
+    sc, _ = small
+    tr, outs, args, *_ = small_run
+    gen = torch.Generator(device="cpu").manual_seed(3)
+    g1 = torch.randn(outs[10].shape, generator=gen).to(device) * 1e-4
+    g2 = torch.randn(outs[10].shape, generator=gen).to(device) * 1e-4
+    a = gpu_backward(tr, sc, outs, args, g1.contiguous())
+    b = gpu_backward(tr, sc, outs, args, g2.contiguous())
+    c = gpu_backward(tr, sc, outs, args, (2 * g1 - 3 * g2).contiguous())
+    for i in (1, 4):
+        assert rel_l2(c[i].cpu().numpy(), (2 * a[i] - 3 * b[i]).cpu().numpy()) < 1e-4
+
+
+@pytest.mark.parametrize("case", ["no_env", "single_gaussian", "miss", "indirect", "one_bounce", "four_bounces"])
+def test_edge_cases_match_oracle(case, device):
+    sc = scenes.small_scene(1, n=1024, res=32, spp=1)
+    tab = env_table(sc)
+    render_opts = 0
+    if case == "no_env":
+        sc.env = torch.zeros(0, 0, 4)
+        tab = None
+    elif case == "single_gaussian":
+        for k in ("positions", "rotation", "scale", "density", "features", "shading_normal", "albedo", "roughness",
+                  "metallic"):
+            setattr(sc, k, getattr(sc, k)[:1].clone())
+        sc.positions.zero_()
+        sc.scale[:] = 0.3
+    elif case == "miss":
+        sc.rays_dir = -sc.rays_dir  # camera looks away from the cloud
+        sc.rays_dir = sc.rays_dir.contiguous()
+    elif case == "indirect":
+        render_opts = 1
+    elif case == "one_bounce":
+        sc.max_bounces = 1
+    elif case == "four_bounces":
+        sc.max_bounces = 4
+    tr = HipTracer(enable_metallic=True)
+    outs, args = gpu_forward(tr, sc, tab, device, render_opts=render_opts)
+    torch.cuda.synchronize()
+    g = to_np(outs)
+    _, _, ref = oracle_forward(sc, tab, render_opts=render_opts)
+    for k in ("dns", "hit", "mat", "pbr", "light"):
+        assert pixel_agreement(g[k], ref[k], RTOL, ATOL) > 0.98, (case, k)
+    if case == "miss":
+        assert float(outs[1].abs().max()) == 0.0
+    if case == "no_env":
+        assert float(outs[10].abs().max()) == 0.0
+    gr = gpu_backward(tr, sc, outs, args, torch.ones_like(outs[10]) * 1e-3, render_opts=render_opts)
+    torch.cuda.synchronize()
+    assert all(bool(torch.isfinite(t).all()) for t in gr)
+
+
+def test_object_scene_crop_matches_oracle(device):
+    full = scenes.object_scene(0)
+    crop = scenes.shrink(full, 48, 48, views=2)
+    tab = env_table(crop)
+    tr = HipTracer(enable_metallic=True)
+    outs, args = gpu_forward(tr, crop, tab, device, counters=True)
+    torch.cuda.synchronize()
+    g = to_np(outs)
+    inp, rays, ref = oracle_forward(crop, tab)
+    for k in ("dns", "hit", "mat"):
+        assert pixel_agreement(g[k], ref[k], RTOL, ATOL) > 0.99, k
+    for k in ("pbr", "light"):
+        assert pixel_agreement(g[k], ref[k], RTOL, ATOL) > 0.97, k
+    gpbr = torch.full_like(outs[10], 1e-3)
+    gr = gpu_backward(tr, crop, outs, args, gpbr)
+    torch.cuda.synchronize()
+    dmat, denv = O.trace_bwd(inp, rays, outs[10].double().cpu().numpy(), outs[11].double().cpu().numpy(),
+                             gpbr.cpu().numpy(), np.zeros(outs[11].shape), np.zeros(outs[7].shape))
+    assert rel_l2(gr[1].cpu().numpy(), dmat) < 5e-3
+    assert rel_l2(gr[4].cpu().numpy(), denv) < 5e-3
+
+
+def test_object_scene_full_size_properties(device):
+    sc = scenes.object_scene(0)
+    tab = env_table(sc)
+    tr = HipTracer(enable_metallic=True)
+    outs, args = gpu_forward(tr, sc, tab, device)
+    torch.cuda.synchronize()
+    for o in outs:
+        assert bool(torch.isfinite(o).all())
+    dns = outs[1]
+    assert float(dns.min()) >= 0.0 and float(dns.max()) < 1.0
+    assert float(outs[10].min()) >= 0.0
+    # the silhouette of the object covers a large part of every view
+    assert 0.2 < float((dns > 0.5).float().mean()) < 0.9
+    # paths with an invalid primary interaction carry no PBR radiance
+    inval = (dns[..., 0] <= 0.5)
+    assert float(outs[10][inval].abs().max()) == 0.0
+    # visibility flags every Gaussian hit; the shell faces the cameras on all sides
+    vis = outs[9].view(torch.int32).reshape(-1)
+    assert 0.05 < float((vis != 0).float().mean()) <= 1.0
+    g = torch.full_like(outs[10], 1e-6)
+    gr = gpu_backward(tr, sc, outs, args, g)
+    torch.cuda.synchronize()
+    assert all(bool(torch.isfinite(t).all()) for t in gr)
+    # hit Gaussians get gradients, never-visible ones none
+    nz = (gr[1].abs().sum(1) > 0)
+    assert bool((~(vis != 0) & nz).sum() == 0)
+
+
+def test_graph_capture_replays_match_eager(small, device):
+    sc, tab = small
+    tr = HipTracer(enable_metallic=True)
+    outs, args = gpu_forward(tr, sc, tab, device)
+    g = torch.full_like(outs[10], 1e-4)
+    eager_f = [o.clone() for o in outs]
+    eager_b = [t.clone() for t in gpu_backward(tr, sc, outs, args, g)]
+    torch.cuda.synchronize()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        # warm-up on the side stream, then capture
+        o2 = tr.trace(*args, render_opts=0, sph_degree=3, min_transmittance=sc.min_transmittance,
+                      max_bounces=sc.max_bounces)
+        gpu_backward(tr, sc, o2, args, g)
+        with torch.cuda.graph(graph, stream=s):
+            cap_f = tr.trace(*args, render_opts=0, sph_degree=3, min_transmittance=sc.min_transmittance,
+                             max_bounces=sc.max_bounces)
+            cap_b = gpu_backward(tr, sc, cap_f, args, g)
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        graph.replay()
+        torch.cuda.synchronize()
+        for a, b in zip(eager_f, cap_f):
+            assert torch.equal(a, b)
+        assert rel_l2(cap_b[1].cpu().numpy(), eager_b[1].cpu().numpy()) < 1e-5
+        assert rel_l2(cap_b[4].cpu().numpy(), eager_b[4].cpu().numpy()) < 1e-5
+        # some eager work between replays
+        _ = (torch.randn(1000, device=device) * 2).sum().item()
