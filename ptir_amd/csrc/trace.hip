@@ -770,8 +770,16 @@ __device__ __forceinline__ void accumulate(const DeviceScene& S, Path& p) {
         if (mis_on && first) p.cur.contribution = p.cur.contribution * bMis;
         p.cur.radiance += p.cur.contribution;
         p.L += p.cur.contribution;
-        if (p.numBounces < 2u) p.direct += p.cur.contribution;
-        else p.indirect += p.cur.contribution;
+        // per-component value selects: a branch between the two members becomes a
+        // pointer select, which keeps the whole path state out of registers
+        const bool isd = p.numBounces < 2u;
+        const V3 c = p.cur.contribution;
+        p.direct.x = isd ? p.direct.x + c.x : p.direct.x;
+        p.direct.y = isd ? p.direct.y + c.y : p.direct.y;
+        p.direct.z = isd ? p.direct.z + c.z : p.direct.z;
+        p.indirect.x = isd ? p.indirect.x : p.indirect.x + c.x;
+        p.indirect.y = isd ? p.indirect.y : p.indirect.y + c.y;
+        p.indirect.z = isd ? p.indirect.z : p.indirect.z + c.z;
     }
 }
 

@@ -161,7 +161,21 @@ def test_edge_cases_match_oracle(case, device):
     assert all(bool(torch.isfinite(t).all()) for t in gr)
 
 
+def _agreeing_paths(g, ref):
+    ok = np.ones(g["pbr"].shape[:3], bool)
+    for k in ("dns", "hit", "mat", "pbr", "light", "comp"):
+        a = g[k].reshape(*g[k].shape[:3], -1)
+        b = ref[k].reshape(a.shape)
+        ok &= (np.abs(a - b) <= ATOL + RTOL * np.abs(b)).all(-1)
+    return ok
+
+
 def test_object_scene_crop_matches_oracle(device):
+    """Object-scale scene (dense thin shell, 500x sun).  About 1% of paths take a
+    different discrete decision in FP32 than in FP64 and differ by a few percent; with a
+    500x sun one such path can dominate a gradient norm.  The backward is linear in the
+    output gradient, so the gradient check masks the output gradient to the paths whose
+    forward outputs agree, and holds those to 1e-3 of the oracle."""
     full = scenes.object_scene(0)
     crop = scenes.shrink(full, 48, 48, views=2)
     tab = env_table(crop)
@@ -170,17 +184,22 @@ def test_object_scene_crop_matches_oracle(device):
     torch.cuda.synchronize()
     g = to_np(outs)
     inp, rays, ref = oracle_forward(crop, tab)
-    for k in ("dns", "hit", "mat"):
-        assert pixel_agreement(g[k], ref[k], RTOL, ATOL) > 0.99, k
-    for k in ("pbr", "light"):
+    for k in ("dns", "hit"):
+        assert pixel_agreement(g[k], ref[k], RTOL, ATOL) > 0.995, k
+    for k in ("mat", "pbr", "light"):
         assert pixel_agreement(g[k], ref[k], RTOL, ATOL) > 0.97, k
-    gpbr = torch.full_like(outs[10], 1e-3)
+    ok = _agreeing_paths(g, ref)
+    assert ok.mean() > 0.97
+    mask = torch.from_numpy(ok.astype(np.float32)).to(device).unsqueeze(-1)
+    gpbr = (torch.full_like(outs[10], 1e-3) * mask).contiguous()
     gr = gpu_backward(tr, crop, outs, args, gpbr)
     torch.cuda.synchronize()
     dmat, denv = O.trace_bwd(inp, rays, outs[10].double().cpu().numpy(), outs[11].double().cpu().numpy(),
                              gpbr.cpu().numpy(), np.zeros(outs[11].shape), np.zeros(outs[7].shape))
-    assert rel_l2(gr[1].cpu().numpy(), dmat) < 5e-3
-    assert rel_l2(gr[4].cpu().numpy(), denv) < 5e-3
+    e_mat, e_env = rel_l2(gr[1].cpu().numpy(), dmat), rel_l2(gr[4].cpu().numpy(), denv)
+    print(f"object crop: {ok.mean():.4f} paths agree; backward rel L2 material {e_mat:.3e} env {e_env:.3e}")
+    assert e_mat < 1e-3
+    assert e_env < 1e-3
 
 
 def test_object_scene_full_size_properties(device):
