@@ -30,6 +30,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 
 # Algorithmic bytes per work unit (see DESIGN.md "Measurement").
 B_CAND, B_ACC, B_SH, B_ENV, B_MAT_ATOM, B_ENV_ATOM, B_IO_FWD, B_IO_BWD = 44, 32, 192, 64, 40, 96, 152, 92
+B_TAPE_PATH, B_TAPE_SEG, B_TAPE_HIT = 64, 64, 8  # path tape: record header, segment record, hit entry
 
 
 def fwd_bytes(c: dict) -> float:
@@ -38,9 +39,16 @@ def fwd_bytes(c: dict) -> float:
 
 
 def bwd_bytes(c: dict) -> float:
+    """Re-tracing backward: the replayed traversal plus the gradient atomics."""
     return (c["candidates"] * B_CAND + c["accepted"] * B_ACC + c["shadow_candidates"] * B_CAND +
             c["env_lookups"] * B_ENV + c["mat_atomics"] * B_MAT_ATOM + c["env_atomics"] * B_ENV_ATOM +
             c["paths"] * B_IO_BWD)
+
+
+def bwd_tape_bytes(c: dict) -> float:
+    """Tape backward: path records and hit entries instead of traversal."""
+    return (c["paths"] * (B_IO_BWD + B_TAPE_PATH) + c["segments"] * B_TAPE_SEG + c["mat_atomics"] * (B_MAT_ATOM + 8) +
+            c["env_atomics"] * B_ENV_ATOM + c["env_lookups"] * B_ENV)
 
 
 def read_traffic(kernel: str):
@@ -87,6 +95,7 @@ def main() -> int:
     ap.add_argument("--kbuffer", type=int, default=16)
     ap.add_argument("--cpu-stride", type=int, default=8, help="oracle path sample stride for cpu_baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-tape", action="store_true", help="backward re-traces every segment (reference behaviour)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only copy a float32 buffer of BYTES bytes 5 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -126,11 +135,14 @@ def main() -> int:
               max_bounces=scene.max_bounces)
     ev = {"fwd": [], "bwd": []}
 
+    tape = None if args.no_tape else tracer.new_tape(g.rays_dir, scene.max_bounces)
+    tk = {} if tape is None else {"tape": tape}
+
     def step(timed=False, counters=False):
         tracer.build_bvh(g.positions, g.rotation, g.scale, g.density)
         e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
         e0.record()
-        outs = tracer.trace(*fargs, counters=counters, **kw)
+        outs = tracer.trace(*fargs, counters=counters, **tk, **kw)
         e1.record()
         cf = tracer.last_counters
         grad = (2.0 / outs[10].numel()) * (outs[10] - target)
@@ -138,7 +150,7 @@ def main() -> int:
         e_b.record()
         tracer.trace_bwd(scene.frame, g.ray_to_world, g.rays_ori, g.rays_dir, *outs[:8], outs[10], outs[11], dens, mat,
                          g.features, g.shading_normal, g.env, tab, None, None, None, None, None, None, None, None,
-                         None, None, None, None, None, None, grad, None, counters=counters, **kw)
+                         None, None, None, None, None, None, grad, None, counters=counters, **tk, **kw)
         e2.record()
         if timed:
             ev["fwd"].append((e0, e1))
@@ -157,8 +169,10 @@ def main() -> int:
     ms_fwd = sum(a.elapsed_time(b) for a, b in ev["fwd"]) / args.steps
     ms_bwd = sum(a.elapsed_time(b) for a, b in ev["bwd"]) / args.steps
     value = paths * args.steps / dt
-    dom, ms_dom, by = ("k_trace_bwd", ms_bwd, bwd_bytes(cb)) if ms_bwd >= ms_fwd else ("k_trace_fwd", ms_fwd,
-                                                                                      fwd_bytes(cf))
+    fb = fwd_bytes(cf) + (0 if tape is None else cf["paths"] * B_TAPE_PATH + cf["segments"] * B_TAPE_SEG +
+                          cf["accepted"] * B_TAPE_HIT)
+    bb = bwd_bytes(cb) if tape is None else bwd_tape_bytes(cb)
+    dom, ms_dom, by = ("k_trace_bwd", ms_bwd, bb) if ms_bwd >= ms_fwd else ("k_trace_fwd", ms_fwd, fb)
     achieved = by / (ms_dom * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": read_traffic(dom),
@@ -171,7 +185,8 @@ def main() -> int:
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "object-scale: 300k Gaussians, 4 views 800x800, 1 spp, maxBounces 3, env 512x256, "
                                "plain tracer (radiance cache not built), ptir-mode backward",
-                   "gaussians": scene.num_gaussians, "paths_per_step": paths, "kbuffer": args.kbuffer},
+                   "gaussians": scene.num_gaussians, "paths_per_step": paths, "kbuffer": args.kbuffer,
+                   "backward": "re-trace" if tape is None else "path tape"},
         "roofline": roofline,
         "work_units": {"fwd": cf, "bwd": cb},
     }

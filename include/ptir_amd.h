@@ -102,6 +102,12 @@ typedef struct ptir_fwd_outputs {
     float* pbr;            /* [..,3] */
     float* light;          /* [..,3] */
     float* pbr_components; /* [..,2,3] direct, indirect */
+    /* Optional path tape (NULL = none): device memory of tape_bytes >= ptir_tape_bytes(...)
+     * into which the forward records, per path, what the backward would otherwise re-trace.
+     * No counterpart in the reference; pass the same buffer to ptir_trace_bwd. */
+    void* tape;
+    uint64_t tape_bytes;
+    uint32_t tape_hits_per_segment;
 } ptir_fwd_outputs;
 
 /* Backward inputs: forward rayPbr/rayLight and the incoming gradients.  The reference's
@@ -113,6 +119,12 @@ typedef struct ptir_bwd_inputs {
     const float* pbr_grad;       /* [..,3] */
     const float* light_grad;     /* [..,3] or NULL */
     const float* material_grad;  /* [..,5] or NULL */
+    /* Optional tape written by the forward of the same launch (NULL = re-trace every
+     * segment, as the reference does).  A tape from a different launch is detected and
+     * ignored; paths whose hits overflowed the tape are re-traced. */
+    const void* tape;
+    uint64_t tape_bytes;
+    uint32_t tape_hits_per_segment;
 } ptir_bwd_inputs;
 
 /* Backward outputs (OptixTracer::traceBwd, optixTracer.cpp:1058-1063).  The library
@@ -156,6 +168,12 @@ ptir_status ptir_trace(ptir_tracer* tracer, const ptir_scene* scene, const ptir_
 ptir_status ptir_trace_bwd(ptir_tracer* tracer, const ptir_scene* scene, const ptir_launch* launch,
                            const ptir_bwd_inputs* in, const ptir_bwd_outputs* out, ptir_counters* counters,
                            void* stream);
+
+/* Bytes of a path tape for a [B,H,W] launch with max_bounces path vertices and room for
+ * hits_per_segment accepted hits per traced segment (0 on invalid sizes).  No counterpart
+ * in the reference: it lets ptir_trace_bwd skip re-tracing (a paths x 16 x 4 B record per
+ * bounce plus 8 B per hit slot). */
+uint64_t ptir_tape_bytes(int32_t B, int32_t H, int32_t W, uint32_t max_bounces, uint32_t hits_per_segment);
 
 /* Human-readable text of the last error on this handle (or of the last ptir_create
  * failure when tracer is NULL).  Valid until the next call on the handle. */

@@ -22,8 +22,8 @@ PTIR_STATUS = {
     6: "PTIR_ERR_OUT_OF_MEMORY",
 }
 
-EXPORTED = ["ptir_create", "ptir_destroy", "ptir_build_bvh", "ptir_trace", "ptir_trace_bwd", "ptir_last_error",
-            "ptir_version"]
+EXPORTED = ["ptir_create", "ptir_destroy", "ptir_build_bvh", "ptir_trace", "ptir_trace_bwd", "ptir_tape_bytes",
+            "ptir_last_error", "ptir_version"]
 
 
 class Options(C.Structure):
@@ -50,11 +50,13 @@ class Launch(C.Structure):
 class FwdOutputs(C.Structure):
     _fields_ = [(k, C.c_void_p) for k in ("radiance", "density", "hit_distance", "hit_distance2", "distortion",
                                           "normal", "shading_normal", "material", "hits_count", "visibility", "pbr",
-                                          "light", "pbr_components")]
+                                          "light", "pbr_components", "tape")] + [
+        ("tape_bytes", C.c_uint64), ("tape_hits_per_segment", C.c_uint32)]
 
 
 class BwdInputs(C.Structure):
-    _fields_ = [(k, C.c_void_p) for k in ("pbr", "light", "pbr_grad", "light_grad", "material_grad")]
+    _fields_ = [(k, C.c_void_p) for k in ("pbr", "light", "pbr_grad", "light_grad", "material_grad", "tape")] + [
+        ("tape_bytes", C.c_uint64), ("tape_hits_per_segment", C.c_uint32)]
 
 
 class BwdOutputs(C.Structure):
@@ -89,6 +91,8 @@ def lib() -> C.CDLL:
                                  C.POINTER(Counters), C.c_void_p]
         L.ptir_trace_bwd.argtypes = [C.c_void_p, C.POINTER(Scene), C.POINTER(Launch), C.POINTER(BwdInputs),
                                      C.POINTER(BwdOutputs), C.POINTER(Counters), C.c_void_p]
+        L.ptir_tape_bytes.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_uint32, C.c_uint32]
+        L.ptir_tape_bytes.restype = C.c_uint64
         L.ptir_last_error.argtypes = [C.c_void_p]
         L.ptir_last_error.restype = C.c_char_p
         L.ptir_version.restype = C.c_char_p

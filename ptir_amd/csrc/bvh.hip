@@ -296,6 +296,7 @@ hipError_t bvh_reserve(BvhBuffers& b, uint32_t n) {
         if ((e = hipMalloc((void**)&b.depth, sizeof(int32_t))) != hipSuccess) return e;
         if ((e = hipMalloc((void**)&b.counters, C_COUNT * sizeof(unsigned long long))) != hipSuccess) return e;
         if ((e = hipMalloc((void**)&b.error_flag, sizeof(int32_t))) != hipSuccess) return e;
+        if ((e = hipMalloc((void**)&b.tape_state, sizeof(int32_t))) != hipSuccess) return e;
     }
     size_t tmp = 0;
     e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, b.keys, b.keys_sorted, b.idx, b.leaf_id, int(n), 0, 63);
@@ -312,7 +313,8 @@ hipError_t bvh_reserve(BvhBuffers& b, uint32_t n) {
 
 void bvh_free(BvhBuffers& b) {
     void* ptrs[] = {b.aabb, b.keys, b.keys_sorted, b.idx, b.leaf_id, b.nodes, b.parent, b.flags, b.bounds_i,
-                    b.bounds, b.depth, b.sort_tmp, b.grec, b.srec, b.prec, b.dmat_leaf, b.counters, b.error_flag};
+                    b.bounds, b.depth, b.sort_tmp, b.grec, b.srec, b.prec, b.dmat_leaf, b.counters, b.error_flag,
+                    b.tape_state};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     b = BvhBuffers();

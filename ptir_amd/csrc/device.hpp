@@ -120,15 +120,50 @@ struct DeviceScene {
     int32_t* error_flag;
 };
 
+// ---------------------------------------------------------------- path tape
+// The forward can record, per path, what the backward replay would otherwise recompute
+// by re-tracing: every segment's accepted hits (leaf slot, weight alpha*T in processing
+// order, cut at the flush bound of pathTracer.cuh:326-329), the segment's transmittance,
+// flush bound, last hit and interaction, the NEE record and the RNG state after NEE.
+// Layout (32-bit words): header[kTapeHeaderWords], then per path a record of
+// kTapePathWords + kTapeSegWords * max_bounces words, then per path and segment
+// `cap` hit entries of 2 words.
+constexpr uint32_t kTapeMagic = 0x50544150u;  // "PTAP"
+constexpr uint32_t kTapeVersion = 1u;
+constexpr int kTapeHeaderWords = 64;
+constexpr int kTapePathWords = 16;
+constexpr int kTapeSegWords = 16;
+constexpr uint32_t kTapeOverflow = 1u;   // path flag: replay this path by re-tracing
+constexpr uint32_t kSegIValid = 1u << 16;
+constexpr uint32_t kSegValid = 1u << 17;
+
+struct Tape {
+    uint32_t* base;     // null: no tape
+    uint32_t cap;       // hit entries per segment
+    uint32_t segs;      // segments per path (= effective max bounces)
+    __device__ __forceinline__ uint32_t rec_words() const { return kTapePathWords + kTapeSegWords * segs; }
+    __device__ __forceinline__ uint32_t* path(size_t pix) const {
+        return base + kTapeHeaderWords + pix * rec_words();
+    }
+    __device__ __forceinline__ uint2* hits(size_t pix, size_t paths, uint32_t seg) const {
+        uint2* h = reinterpret_cast<uint2*>(base + kTapeHeaderWords + paths * rec_words());
+        return h + (pix * segs + seg) * cap;
+    }
+};
+
 struct FwdOut {
     float *rad, *dns, *hit, *hit2, *dist, *nrm, *snrm, *mat, *hits, *pbr, *light, *comp;
     uint32_t* vis;  // [N] original order
+    Tape tape;
 };
 
 struct BwdIO {
     const float *pbr, *light, *pbr_grad, *light_grad, *mat_grad;
     float* dmat_leaf;  // [N,5] leaf order (scattered to the original order afterwards)
     float* denv;       // [env_h, env_w, 4]
+    Tape tape;
+    int32_t* tape_state;  // [1]: 0 = tape valid, 1 = header mismatch (replay every path)
+    int32_t replay_mode;  // 0: all paths, 1: only paths the tape kernel could not handle
 };
 
 enum CounterId { C_PATHS = 0, C_SEGMENTS, C_CANDIDATES, C_ACCEPTED, C_ACCEPTED_PRIMARY, C_SHADOW_CANDIDATES,

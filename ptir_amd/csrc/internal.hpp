@@ -37,6 +37,7 @@ struct BvhBuffers {
     float* dmat_leaf = nullptr;      // [5N]
     unsigned long long* counters = nullptr;  // [C_COUNT]
     int32_t* error_flag = nullptr;   // [1]
+    int32_t* tape_state = nullptr;   // [1]
 };
 
 hipError_t bvh_reserve(BvhBuffers& b, uint32_t n);
@@ -55,5 +56,11 @@ hipError_t launch_trace_bwd(const DeviceScene& S, const BwdIO& io, int kbuf, hip
 
 constexpr int kTraceBlock = 128;  // 2 waves: a 16x8 pixel tile
 constexpr int kStackDepth = 64;   // traversal stack entries per lane (LDS)
+
+// bytes of a tape with `segs` segments per path and `cap` hits per segment
+inline uint64_t tape_bytes(uint64_t paths, uint32_t segs, uint32_t cap) {
+    const uint64_t rec = uint64_t(kTapePathWords) + uint64_t(kTapeSegWords) * segs;
+    return 4ull * (kTapeHeaderWords + paths * rec) + 8ull * paths * segs * cap;
+}
 
 }  // namespace ptir

@@ -184,6 +184,8 @@ struct Path {  // payLoad.h:142-168
     float popacity, pmaxHit;
     V3 pdir;
     unsigned pbounces;
+    // tape segment index and hit count of the current and of the pending segment
+    uint32_t cseg, ccnt, pseg, pcnt;
 };
 
 __device__ __forceinline__ void emit_clear(Path& p) {
@@ -552,9 +554,18 @@ __device__ __forceinline__ V3 sh_radiance(const DeviceScene& S, uint32_t id, V3 
 }
 
 // pathTracer.cuh:216-311 rayIntersect with gaussianParticles.cuh:343-440 processHit
+// Tape writer of one segment: accepted hits in processing order.
+struct SegTape {
+    uint2* hits;     // null: do not record
+    uint32_t cap, n;
+    bool overflow;
+    float tlast, tprev;  // distances of the last two recorded hits
+    float endT;          // flush bound of the backward re-trace
+};
+
 template <int K>
 __device__ __forceinline__ void ray_intersect(const DeviceScene& S, const Ray ray, Seg& pl, Rng& rng, bool selfOcc, bool primary,
-                              bool gbuffer, uint32_t* vis, const Lane& L) {
+                              bool gbuffer, uint32_t* vis, const Lane& L, SegTape& tp) {
     float t0, t1;
     scene_interval(S, ray, t0, t1);
     const float sPdf = pl.scatterPdf, lPdf = pl.lightPdf;
@@ -606,6 +617,16 @@ __device__ __forceinline__ void ray_intersect(const DeviceScene& S, const Ray ra
                         pl.normal += w * safe_normalize((h.gro + h.grd * (proj - sqrtf(9.f - h.gd))) * xyz(rs));
                     }
                 }
+                if (tp.hits) {
+                    if (tp.n < tp.cap) {
+                        tp.hits[tp.n] = make_uint2(uint32_t(slot), __float_as_uint(w));
+                        ++tp.n;
+                        tp.tprev = tp.tlast;
+                        tp.tlast = t;
+                    } else {
+                        tp.overflow = true;
+                    }
+                }
                 pl.hit = true;
                 const float op = 1.f - pl.T;
                 pl.distortion = fmaxf(op * pl.hitDist2 - depth * depth, 0.f);
@@ -615,6 +636,15 @@ __device__ __forceinline__ void ray_intersect(const DeviceScene& S, const Ray ra
             }
             pl.lastHit = fmaxf(pl.lastHit, t);
             if (S.enable_hitcounts) pl.hits += acc ? 1.f : 0.f;
+        }
+    }
+    if (tp.hits) {
+        // the backward flush re-traces (tmin, min(lastHit, t1) + 1e-9f) (pathTracer.cuh:326-329):
+        // in float32 the bound usually equals lastHit, which drops the last processed hit
+        tp.endT = fminf(pl.lastHit, t1) + kEpsT;
+        if (tp.n > 0 && tp.tlast >= tp.endT) {
+            --tp.n;
+            if (tp.n > 0 && tp.tprev >= tp.endT) tp.overflow = true;  // 3-way tie: the replay handles it
         }
     }
     const float op = 1.f - pl.T;
@@ -850,13 +880,52 @@ __device__ __forceinline__ void sample_next(const DeviceScene& S, Path& p, Rng& 
     }
 }
 
-template <int K>
-__device__ __forceinline__ void pending_bwd(const DeviceScene& S, Path& p, float* dmat, const Lane& L) {
+// Where the backward's material-gradient flushes come from: a re-trace of the segment
+// (pathTracer.cuh:313-390) or, with a tape, the segment's recorded hits.  Both give the
+// same per-hit products w * (g / opacity), with w = alpha * T computed identically.
+struct Flush {
+    const BwdIO* io;
+    size_t pix, paths;
+    const Lane* L;
+};
+
+__device__ __forceinline__ void tape_flush(const DeviceScene& S, const Flush& f, uint32_t seg, uint32_t cnt, float opacity,
+                                           V3 ga, float gr, float gm) {
+    const float inv = 1.f / fmaxf(opacity, 1e-12f);
+    ga = ga * inv;
+    gr *= inv;
+    gm *= inv;
+    const uint2* h = f.io->tape.hits(f.pix, f.paths, seg);
+    for (uint32_t i = 0; i < cnt; ++i) {
+        const uint2 e = h[i];
+        const float w = __uint_as_float(e.y);
+        float* q = f.io->dmat_leaf + 5 * size_t(e.x);
+        atomicAdd(q + 0, w * ga.x);
+        atomicAdd(q + 1, w * ga.y);
+        atomicAdd(q + 2, w * ga.z);
+        atomicAdd(q + 3, w * gr);
+        if (S.enable_metallic) atomicAdd(q + 4, w * gm);
+        count(S, C_MAT_ATOMICS);
+    }
+}
+
+template <int K, bool TAPE>
+__device__ __forceinline__ void flush_seg(const DeviceScene& S, const Flush& f, const Ray& ray, float opacity, float maxHit,
+                                          uint32_t seg, uint32_t cnt, V3 ga, float gr, float gm, bool selfOcc) {
+    if constexpr (TAPE) {
+        tape_flush(S, f, seg, cnt, opacity, ga, gr, gm);
+    } else {
+        intersect_bwd<K>(S, ray, opacity, maxHit, ga, gr, gm, selfOcc, f.io->dmat_leaf, *f.L);
+    }
+}
+
+template <int K, bool TAPE>
+__device__ __forceinline__ void pending_bwd(const DeviceScene& S, Path& p, const Flush& f) {
     if (!p.pvalid) return;
     const float dR = dot(p.cur.dirGrad, p.pdir);
     p.pvalid = false;
     if (dR == 0.f) return;
-    intersect_bwd<K>(S, p.pray, p.popacity, p.pmaxHit, v3s(0.f), dR, 0.f, p.pbounces > 1u, dmat, L);
+    flush_seg<K, TAPE>(S, f, p.pray, p.popacity, p.pmaxHit, p.pseg, p.pcnt, v3s(0.f), dR, 0.f, p.pbounces > 1u);
 }
 
 __device__ __forceinline__ void nee_grad_bwd(const DeviceScene& S, Path& p, const Seg& cs) {
@@ -874,8 +943,8 @@ __device__ __forceinline__ void nee_grad_bwd(const DeviceScene& S, Path& p, cons
     p.cur.galb += dLdB * bc.dAlb;
 }
 
-template <int K>
-__device__ __forceinline__ void sample_next_bwd(const DeviceScene& S, Path& p, Rng& rng, float* dmat, const Lane& L) {
+template <int K, bool TAPE>
+__device__ __forceinline__ void sample_next_bwd(const DeviceScene& S, Path& p, Rng& rng, const Flush& f) {
     V3 n, wo, alb;
     float met;
     shading_frame(S, p.cur, n, wo, alb, met);
@@ -913,8 +982,10 @@ __device__ __forceinline__ void sample_next_bwd(const DeviceScene& S, Path& p, R
     p.pmaxHit = p.cur.lastHit;
     p.pdir = bt.dDir;
     p.pbounces = p.numBounces;
-    intersect_bwd<K>(S, p.cur.ray, 1.f - p.cur.T, p.cur.lastHit, p.cur.galb, p.cur.grough, p.cur.gmetal,
-                     p.numBounces > 1u, dmat, L);
+    p.pseg = p.cseg;
+    p.pcnt = p.ccnt;
+    flush_seg<K, TAPE>(S, f, p.cur.ray, 1.f - p.cur.T, p.cur.lastHit, p.cseg, p.ccnt, p.cur.galb, p.cur.grough,
+                       p.cur.gmetal, p.numBounces > 1u);
     p.thr = nt;
     const Ray nr{p.cur.pos + safe_normalize(Ld) * kSelfOccOffset, Ld};
     p.cur.reset(nr, 0.f);
@@ -922,6 +993,45 @@ __device__ __forceinline__ void sample_next_bwd(const DeviceScene& S, Path& p, R
         p.cur.scatterPdf = sPdf;
         p.cur.lightPdf = lPdf;
     }
+}
+
+// ---------------------------------------------------------------- tape records
+__device__ __forceinline__ void store_seg(uint32_t* rec, uint32_t s, const Seg& pl, const SegTape& tp) {
+    float4* q = reinterpret_cast<float4*>(rec + kTapePathWords + kTapeSegWords * s);
+    const uint32_t packed = (tp.n & 0xFFFFu) | (pl.ivalid ? kSegIValid : 0u) | (pl.valid ? kSegValid : 0u);
+    q[0] = make_float4(pl.T, tp.endT, pl.lastHit, __uint_as_float(packed));
+    q[1] = make_float4(pl.pos.x, pl.pos.y, pl.pos.z, pl.sn.x);
+    q[2] = make_float4(pl.sn.y, pl.sn.z, pl.alb.x, pl.alb.y);
+    q[3] = make_float4(pl.alb.z, pl.rough, pl.metal, 0.f);
+}
+
+// Segment s from the tape, traced along `ray`; keeps the MIS pdfs already set on pl.
+__device__ __forceinline__ uint32_t load_seg(const uint32_t* rec, uint32_t s, const Ray& ray, Seg& pl) {
+    const float4* q = reinterpret_cast<const float4*>(rec + kTapePathWords + kTapeSegWords * s);
+    const float4 a = q[0], b = q[1], c = q[2], d = q[3];
+    const float sPdf = pl.scatterPdf, lPdf = pl.lightPdf;
+    pl.reset(ray, a.z);
+    pl.scatterPdf = sPdf;
+    pl.lightPdf = lPdf;
+    const uint32_t packed = __float_as_uint(a.w);
+    pl.T = a.x;
+    pl.ivalid = (packed & kSegIValid) != 0u;
+    pl.valid = (packed & kSegValid) != 0u;
+    pl.pos = v3(b.x, b.y, b.z);
+    pl.sn = v3(b.w, c.x, c.y);
+    pl.alb = v3(c.z, c.w, d.x);
+    pl.rough = d.y;
+    pl.metal = d.z;
+    return packed & 0xFFFFu;
+}
+
+__device__ __forceinline__ void seg_tape_begin(SegTape& tp, const Tape& t, size_t pix, size_t paths, uint32_t seg) {
+    tp.hits = t.hits(pix, paths, seg);
+    tp.cap = t.cap;
+    tp.n = 0;
+    tp.overflow = false;
+    tp.tlast = tp.tprev = -__int_as_float(0x7f800000);
+    tp.endT = 0.f;
 }
 
 // ---------------------------------------------------------------- launch mapping
@@ -972,6 +1082,7 @@ __device__ __forceinline__ void path_init(Path& p) {
     p.pbounces = 0u;
     p.popacity = p.pmaxHit = 0.f;
     p.pdir = v3s(0.f);
+    p.cseg = p.ccnt = p.pseg = p.pcnt = 0u;
     p.cur.reset(Ray{v3s(0.f), v3s(0.f)}, 0.f);
 }
 
@@ -983,12 +1094,28 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_fwd(DeviceScene S, FwdOut
     const PixelId px = pixel_of(S);
     if (!px.ok) return;
     const size_t pix = (size_t(px.z) * S.H + px.y) * S.W + px.x;
+    const size_t paths = size_t(S.B) * S.H * S.W;
+    const bool rec = out.tape.base != nullptr;
+    uint32_t* prec = rec ? out.tape.path(pix) : nullptr;
+    if (rec && pix == 0) {
+        uint32_t* h = out.tape.base;
+        h[0] = kTapeMagic; h[1] = kTapeVersion; h[2] = uint32_t(S.B); h[3] = uint32_t(S.H); h[4] = uint32_t(S.W);
+        h[5] = out.tape.segs; h[6] = S.n; h[7] = S.frame; h[8] = out.tape.cap;
+    }
+    uint32_t flags = 0u;
+    SegTape tp;
+    tp.hits = nullptr;
     Rng rng;
     rng.seed(Rng::launch_seed(uint32_t(px.x), uint32_t(px.y), uint32_t(px.z), S.frame));
     Path p;
     path_init(p);
     count(S, C_PATHS);
-    ray_intersect<K>(S, primary_ray(S, pix), p.cur, rng, false, true, true, out.vis, L);
+    if (rec) seg_tape_begin(tp, out.tape, pix, paths, 0);
+    ray_intersect<K>(S, primary_ray(S, pix), p.cur, rng, false, true, true, out.vis, L, tp);
+    if (rec) {
+        store_seg(prec, 0, p.cur, tp);
+        flags |= tp.overflow ? kTapeOverflow : 0u;
+    }
     {
         const Seg& c = p.cur;
         float* o;
@@ -1004,6 +1131,13 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_fwd(DeviceScene S, FwdOut
     }
     if (p.cur.ivalid) {
         if (S.enable_mis) sample_nee<K>(S, p, rng, L);
+        if (rec) {
+            prec[1] = uint32_t(rng.state);
+            prec[2] = uint32_t(rng.state >> 32);
+            float4* e = reinterpret_cast<float4*>(prec + 4);
+            e[0] = make_float4(p.eray.d.x, p.eray.d.y, p.eray.d.z, p.elightPdf);
+            e[1] = make_float4(p.escatterPdf, p.econtrib.x, p.econtrib.y, p.econtrib.z);
+        }
         for (unsigned depth = 0; depth < S.max_bounces && p.active; ++depth) {
             accumulate(S, p);
             p.active &= (depth + 1u < S.max_bounces) && p.cur.ivalid;
@@ -1011,9 +1145,15 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_fwd(DeviceScene S, FwdOut
             p.numBounces = depth + 1u;
             sample_next(S, p, rng);
             if (vmax(p.thr) < 1e-4f) break;
-            ray_intersect<K>(S, p.cur.ray, p.cur, rng, true, false, true, out.vis, L);
+            if (rec) seg_tape_begin(tp, out.tape, pix, paths, p.numBounces);
+            ray_intersect<K>(S, p.cur.ray, p.cur, rng, true, false, true, out.vis, L, tp);
+            if (rec) {
+                store_seg(prec, p.numBounces, p.cur, tp);
+                flags |= tp.overflow ? kTapeOverflow : 0u;
+            }
         }
     }
+    if (rec) prec[0] = flags;
     float* o;
     o = out.pbr + 3 * pix; o[0] = p.L.x; o[1] = p.L.y; o[2] = p.L.z;
     o = out.light + 3 * pix; o[0] = p.Lnb.x; o[1] = p.Lnb.y; o[2] = p.Lnb.z;
@@ -1022,14 +1162,29 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_fwd(DeviceScene S, FwdOut
     o[3] = p.indirect.x; o[4] = p.indirect.y; o[5] = p.indirect.z;
 }
 
-// referenceBwdOptix.cu:27-74
-template <int K>
+// referenceBwdOptix.cu:27-74.  TAPE = false replays the path by re-tracing every
+// segment; TAPE = true reads the segments and their hits from the forward's tape.
+template <int K, bool TAPE>
 __global__ void __launch_bounds__(kTraceBlock) k_trace_bwd(DeviceScene S, BwdIO io) {
-    __shared__ uint32_t lds[LdsLayout<K>::kWords * kTraceBlock];
+    __shared__ uint32_t lds[(TAPE ? 1 : LdsLayout<K>::kWords) * kTraceBlock];
     const Lane L{lds + threadIdx.x};
     const PixelId px = pixel_of(S);
     if (!px.ok) return;
     const size_t pix = (size_t(px.z) * S.H + px.y) * S.W + px.x;
+    const size_t paths = size_t(S.B) * S.H * S.W;
+    const uint32_t* prec = io.tape.base ? io.tape.path(pix) : nullptr;
+    if constexpr (TAPE) {
+        const uint32_t* h = io.tape.base;
+        if (h[0] != kTapeMagic || h[1] != kTapeVersion || h[2] != uint32_t(S.B) || h[3] != uint32_t(S.H) ||
+            h[4] != uint32_t(S.W) || h[5] != io.tape.segs || h[6] != S.n || h[7] != S.frame || h[8] != io.tape.cap) {
+            atomicOr(io.tape_state, 1);  // the replay launch that follows handles every path
+            return;
+        }
+        if (prec[0] & kTapeOverflow) return;
+    } else {
+        if (io.replay_mode == 1 && *io.tape_state == 0 && !(prec[0] & kTapeOverflow)) return;
+    }
+    const Flush f{&io, pix, paths, &L};
     Rng rng;
     rng.seed(Rng::launch_seed(uint32_t(px.x), uint32_t(px.y), uint32_t(px.z), S.frame));
     Path p;
@@ -1039,7 +1194,14 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_bwd(DeviceScene S, BwdIO 
     p.dL = v3(io.pbr_grad[3 * pix], io.pbr_grad[3 * pix + 1], io.pbr_grad[3 * pix + 2]);
     p.Lnb = v3(io.light[3 * pix], io.light[3 * pix + 1], io.light[3 * pix + 2]);
     if (io.light_grad) p.dLnb = v3(io.light_grad[3 * pix], io.light_grad[3 * pix + 1], io.light_grad[3 * pix + 2]);
-    ray_intersect<K>(S, primary_ray(S, pix), p.cur, rng, false, true, false, nullptr, L);
+    if constexpr (TAPE) {
+        p.ccnt = load_seg(prec, 0, primary_ray(S, pix), p.cur);
+        count(S, C_SEGMENTS);
+    } else {
+        SegTape none;
+        none.hits = nullptr;
+        ray_intersect<K>(S, primary_ray(S, pix), p.cur, rng, false, true, false, nullptr, L, none);
+    }
     if (!p.cur.ivalid) return;
     if (io.mat_grad) {
         const float* mg = io.mat_grad + 5 * pix;
@@ -1047,16 +1209,38 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_bwd(DeviceScene S, BwdIO 
         p.cur.grough += mg[3];
         if (S.enable_metallic) p.cur.gmetal += mg[4];
     }
-    if (S.enable_mis) sample_nee<K>(S, p, rng, L);
+    if constexpr (TAPE) {
+        // the NEE record and the RNG state after it (shadow-ray draws included)
+        rng.state = uint64_t(prec[1]) | (uint64_t(prec[2]) << 32);
+        if (S.enable_mis) {
+            const float4* e = reinterpret_cast<const float4*>(prec + 4);
+            const float4 e0 = e[0], e1 = e[1];
+            p.eray.d = v3(e0.x, e0.y, e0.z);
+            p.elightPdf = e0.w;
+            p.escatterPdf = e1.x;
+            p.econtrib = v3(e1.y, e1.z, e1.w);
+        }
+    } else {
+        if (S.enable_mis) sample_nee<K>(S, p, rng, L);
+    }
     for (unsigned depth = 0; depth < S.max_bounces && p.active; ++depth) {
         accumulate_bwd(S, p, io.denv);
-        pending_bwd<K>(S, p, io.dmat_leaf, L);
+        pending_bwd<K, TAPE>(S, p, f);
         p.active &= (depth + 1u < S.max_bounces) && p.cur.ivalid;
         if (!p.active) break;
         p.numBounces = depth + 1u;
-        sample_next_bwd<K>(S, p, rng, io.dmat_leaf, L);
+        sample_next_bwd<K, TAPE>(S, p, rng, f);
         if (vmax(p.thr) < 1e-4f) break;
-        ray_intersect<K>(S, p.cur.ray, p.cur, rng, true, false, false, nullptr, L);
+        if constexpr (TAPE) {
+            p.cseg = p.numBounces;
+            p.ccnt = load_seg(prec, p.numBounces, p.cur.ray, p.cur);
+            count(S, C_SEGMENTS);
+            if (!p.cur.ivalid) (void)rng.next_1d();  // the forward's escape draw
+        } else {
+            SegTape none;
+            none.hits = nullptr;
+            ray_intersect<K>(S, p.cur.ray, p.cur, rng, true, false, false, nullptr, L, none);
+        }
     }
 }
 
@@ -1082,10 +1266,24 @@ hipError_t launch_trace_bwd(const DeviceScene& S0, const BwdIO& io, int kbuf, hi
     DeviceScene S = S0;
     tiling(S);
     if (S.num_blocks == 0) return hipSuccess;
+    if (io.tape.base) {
+        // tape pass for every path it covers, then a re-trace pass for the rest
+        k_trace_bwd<16, true><<<S.num_blocks, kTraceBlock, 0, s>>>(S, io);
+        BwdIO r = io;
+        r.replay_mode = 1;
+        switch (kbuf) {
+        case 8: k_trace_bwd<8, false><<<S.num_blocks, kTraceBlock, 0, s>>>(S, r); break;
+        case 32: k_trace_bwd<32, false><<<S.num_blocks, kTraceBlock, 0, s>>>(S, r); break;
+        default: k_trace_bwd<16, false><<<S.num_blocks, kTraceBlock, 0, s>>>(S, r); break;
+        }
+        return hipGetLastError();
+    }
+    BwdIO r = io;
+    r.replay_mode = 0;
     switch (kbuf) {
-    case 8: k_trace_bwd<8><<<S.num_blocks, kTraceBlock, 0, s>>>(S, io); break;
-    case 32: k_trace_bwd<32><<<S.num_blocks, kTraceBlock, 0, s>>>(S, io); break;
-    default: k_trace_bwd<16><<<S.num_blocks, kTraceBlock, 0, s>>>(S, io); break;
+    case 8: k_trace_bwd<8, false><<<S.num_blocks, kTraceBlock, 0, s>>>(S, r); break;
+    case 32: k_trace_bwd<32, false><<<S.num_blocks, kTraceBlock, 0, s>>>(S, r); break;
+    default: k_trace_bwd<16, false><<<S.num_blocks, kTraceBlock, 0, s>>>(S, r); break;
     }
     return hipGetLastError();
 }

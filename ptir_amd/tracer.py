@@ -56,7 +56,8 @@ class HipTracer:
                  particle_kernel_degree: float = 4.0, particle_kernel_min_response: float = 0.0113,
                  particle_kernel_density_clamping: bool = True, particle_radiance_sph_degree: int = 3,
                  enable_normals: bool = True, enable_hitcounts: bool = True, enable_mis: bool = True,
-                 enable_metallic: bool = False, visualize_lights: bool = False, kbuffer: int = 16):
+                 enable_metallic: bool = False, visualize_lights: bool = False, kbuffer: int = 16,
+                 use_tape: bool = True, tape_hits_per_segment: int = 32):
         if pipeline != "reference" or backward_pipeline not in ("referenceBwd", "reference" + "Bwd"):
             raise ValueError(f"unsupported pipeline pair ({pipeline}, {backward_pipeline}); "
                              "only the PTIR 'reference' path tracer is built")
@@ -73,6 +74,21 @@ class HipTracer:
         self.options = o
         self.sph_max_degree = int(particle_radiance_sph_degree)
         self.last_counters: dict | None = None
+        # the autograd function records a path tape in the forward so that the backward
+        # replays paths without re-tracing them (same gradients, no BVH traversal)
+        self.use_tape = bool(use_tape)
+        self.tape_hits_per_segment = int(tape_hits_per_segment)
+
+    def tape_bytes(self, B: int, H: int, W: int, max_bounces: int) -> int:
+        return int(_lib.lib().ptir_tape_bytes(B, H, W, max_bounces, self.tape_hits_per_segment))
+
+    def new_tape(self, ray_dir: torch.Tensor, max_bounces: int) -> torch.Tensor:
+        """Device buffer for the path tape of one trace call on rays of ray_dir's shape."""
+        B, H, W = ray_dir.shape[:3]
+        n = self.tape_bytes(B, H, W, int(max_bounces))
+        if n == 0:
+            raise ValueError("invalid launch size for a tape")
+        return torch.empty(n, dtype=torch.uint8, device=ray_dir.device)
 
     def close(self) -> None:
         """Free the device buffers of this tracer (also done on garbage collection)."""
@@ -157,7 +173,7 @@ class HipTracer:
               particle_radiance, particle_shading_normal, environment, environment_alias_table, lights=None,
               light_alias_table=None, mesh_light_vertices=None, mesh_light_triangles=None, mesh_lights=None,
               mesh_light_triangle_alias_table=None, render_opts=0, sph_degree=3, min_transmittance=1e-3,
-              max_bounces=3, counters=False):
+              max_bounces=3, counters=False, tape: torch.Tensor | None = None):
         """OptixTracer::trace (optixTracer.cpp:909-1013): returns the 13 outputs
         (rad, dns, hit[2], hit2, distortion, nrm, snrm, material, hits, visibility, pbr,
         light, pbr_components)."""
@@ -182,6 +198,9 @@ class HipTracer:
         light = torch.empty(B, H, W, 3, **f)
         comp = torch.empty(B, H, W, 2, 3, **f)
         out = _lib.FwdOutputs(*[_ptr(t) for t in (rad, dns, hit, hit2, dist, nrm, snrm, mat, hits, vis, pbr, light, comp)])
+        if tape is not None:
+            tape = _need(tape, "tape", dtype=torch.uint8)
+            out.tape, out.tape_bytes, out.tape_hits_per_segment = _ptr(tape), tape.numel(), self.tape_hits_per_segment
         cnt = _lib.Counters() if counters else None
         st = _lib.lib().ptir_trace(self._h, C.byref(sc), C.byref(la), C.byref(out),
                                    C.byref(cnt) if cnt is not None else None, _stream())
@@ -198,7 +217,7 @@ class HipTracer:
                   light_alias_table, mesh_light_vertices, mesh_light_triangles, mesh_lights,
                   mesh_light_triangle_alias_table, ray_rad_grd, ray_dns_grd, ray_hit_grd, ray_hit2_grd, ray_dist_grd,
                   ray_nrm_grd, ray_snrm_grd, ray_mat_grd, ray_pbr_grd, ray_light_grd, render_opts=0, sph_degree=3,
-                  min_transmittance=1e-3, max_bounces=3, counters=False):
+                  min_transmittance=1e-3, max_bounces=3, counters=False, tape: torch.Tensor | None = None):
         """OptixTracer::traceBwd (optixTracer.cpp:1015-1145): returns (dDensity [N,12],
         dMaterial [N,5], dSH, dShadingNormal [N,3], dEnv [H,W,4])."""
         sc, keep, env = self._scene(particle_density, particle_material, particle_radiance, particle_shading_normal,
@@ -230,6 +249,9 @@ class HipTracer:
         d_sn = torch.empty(n, particle_shading_normal.shape[1], **f)
         d_env = torch.empty_like(environment) if environment is not None else torch.zeros(0, 0, 4, **f)
         bi = _lib.BwdInputs(_ptr(pbr), _ptr(light), _ptr(pbr_g), _ptr(light_g), _ptr(mat_g))
+        if tape is not None:
+            tape = _need(tape, "tape", dtype=torch.uint8)
+            bi.tape, bi.tape_bytes, bi.tape_hits_per_segment = _ptr(tape), tape.numel(), self.tape_hits_per_segment
         bo = _lib.BwdOutputs(_ptr(d_dns), _ptr(d_mat), _ptr(d_sh), _ptr(d_sn), _ptr(d_env) if env is not None else None)
         if env is None and d_env.numel():
             d_env.zero_()
@@ -317,12 +339,17 @@ class Tracer:
             particle_material = torch.concat([mog_malb, mog_mrgh, mog_mmet], dim=1)
             light_buffers = light_buffers.to(ray_dir.device)
             lights, lat, mlv, mlt, ml, mlta = light_buffers.tensors()
+            tape = None
+            if getattr(tracer_wrapper, "use_tape", False) and any(ctx.needs_input_grad):
+                tape = tracer_wrapper.new_tape(ray_dir, max_bounces)
             (ray_radiance, ray_density, ray_hit_distance, ray_hit_distance_second_moment, ray_depth_distortion,
              ray_normals, ray_shadingnormal, ray_material, hits_count, mog_visibility, ray_pbr, ray_light,
              pbr_components) = tracer_wrapper.trace(frame_id, ray_to_world, ray_ori, ray_dir, particle_density,
                                                     particle_material, mog_sph, mog_snrm, environment,
                                                     environment_alias_table, lights, lat, mlv, mlt, ml, mlta,
-                                                    render_opts, sph_degree, min_transmittance, max_bounces)
+                                                    render_opts, sph_degree, min_transmittance, max_bounces,
+                                                    **({"tape": tape} if tape is not None else {}))
+            ctx.tape = tape
             ctx.save_for_backward(ray_to_world, ray_ori, ray_dir, ray_radiance, ray_density, ray_hit_distance,
                                   ray_hit_distance_second_moment, ray_depth_distortion, ray_normals,
                                   ray_shadingnormal, ray_material, ray_pbr, ray_light, particle_density,
@@ -361,7 +388,9 @@ class Tracer:
                 environment_alias_table, lights, lat, mlv, mlt, ml, mlta, ray_radiance_grd, ray_density_grd,
                 ray_hit_distance_grd, ray_hit_distance_second_moment_grd, ray_depth_distortion_grd, ray_normals_grd,
                 ray_shadingnormal_grd, ray_material_grd, ray_pbr_grd.contiguous(), ray_light_grd.contiguous(),
-                ctx.render_opts, ctx.sph_degree, ctx.min_transmittance, ctx.max_bounces)
+                ctx.render_opts, ctx.sph_degree, ctx.min_transmittance, ctx.max_bounces,
+                **({"tape": ctx.tape} if ctx.tape is not None else {}))
+            ctx.tape = None
             mog_pos_grd, mog_dns_grd, mog_rot_grd, mog_scl_grd, _ = torch.split(particle_density_grd, [3, 1, 4, 3, 1],
                                                                                 dim=1)
             mog_malb_grd, mog_mrgh_grd, mog_mmet_grd = torch.split(particle_material_grd, [3, 1, 1], dim=1)

@@ -17,7 +17,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def _declared_symbols():
     text = open(os.path.join(ROOT, "include", "ptir_amd.h")).read()
-    return sorted(set(re.findall(r"^(?:ptir_status|const char\*)\s+(ptir_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^(?:ptir_status|const char\*|uint64_t)\s+(ptir_\w+)\s*\(", text, re.M)))
 
 
 def test_header_declares_the_abi():
@@ -36,6 +36,14 @@ def _opts(**kw):
              enable_hitcounts=1, enable_mis=1, enable_metallic=0, visualize_lights=0, kbuffer=16)
     o.update(kw)
     return _lib.Options(**o)
+
+
+def test_tape_size_formula():
+    L = _lib.lib()
+    # header 64 words + per path (16 + 16 * segments) words + 8 bytes per hit slot
+    assert L.ptir_tape_bytes(2, 3, 4, 3, 32) == 4 * (64 + 24 * (16 + 48)) + 8 * 24 * 3 * 32
+    assert L.ptir_tape_bytes(-1, 3, 4, 3, 32) == 0
+    assert L.ptir_tape_bytes(1, 1, 1, 3, 0) == 0
 
 
 def test_create_destroy_and_option_errors():

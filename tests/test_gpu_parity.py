@@ -261,3 +261,77 @@ def test_graph_capture_replays_match_eager(small, device):
         assert rel_l2(cap_b[4].cpu().numpy(), eager_b[4].cpu().numpy()) < 1e-5
         # some eager work between replays
         _ = (torch.randn(1000, device=device) * 2).sum().item()
+
+
+# ----------------------------------------------------------------------------- path tape
+@pytest.mark.parametrize("cap", [32, 2])
+def test_tape_backward_equals_replay(small, device, cap):
+    """The tape backward reads the forward's recorded hits instead of re-tracing; it must
+    give the replay's gradients (up to the order of float atomics).  cap=2 forces most
+    paths to overflow their tape and take the re-trace fallback."""
+    sc, tab = small
+    tr = HipTracer(enable_metallic=True, tape_hits_per_segment=cap)
+    g = sc.to(device)
+    tr.build_bvh(g.positions, g.rotation, g.scale, g.density)
+    args = (sc.frame, g.ray_to_world, g.rays_ori, g.rays_dir, g.particle_density(), g.particle_material(), g.features,
+            g.shading_normal, g.env, tab.to(device))
+    kw = dict(render_opts=0, sph_degree=3, min_transmittance=sc.min_transmittance, max_bounces=sc.max_bounces)
+    tape = tr.new_tape(g.rays_dir, sc.max_bounces)
+    outs = tr.trace(*args, tape=tape, **kw)
+    plain = tr.trace(*args, **kw)
+    for a, b in zip(outs, plain):  # recording a tape does not change the forward
+        assert torch.equal(a, b)
+    gp = (torch.rand(outs[10].shape, generator=torch.Generator().manual_seed(4)).to(device) * 1e-3).contiguous()
+    mg = torch.full_like(outs[7], 1e-5)
+    lg = torch.full_like(outs[11], 1e-6)
+    bw = dict(ray_mat_grd=mg, ray_pbr_grd=gp, ray_light_grd=lg, **kw)
+    common = (sc.frame, g.ray_to_world, g.rays_ori, g.rays_dir, *outs[:8], outs[10], outs[11], *args[4:], None, None,
+              None, None, None, None, None, None, None, None, None, None, None)
+    with_tape = tr.trace_bwd(*common, tape=tape, counters=True, **bw)
+    n_tape = tr.last_counters
+    replay = tr.trace_bwd(*common, counters=True, **bw)
+    torch.cuda.synchronize()
+    for i in (1, 4):
+        assert rel_l2(with_tape[i].cpu().numpy(), replay[i].cpu().numpy()) < 1e-5
+    assert n_tape["mat_atomics"] == tr.last_counters["mat_atomics"]
+    assert n_tape["env_atomics"] == tr.last_counters["env_atomics"]
+    # a tape recorded for another frame is detected and ignored (every path re-traced)
+    other = tr.new_tape(g.rays_dir, sc.max_bounces)
+    tr.trace(sc.frame + 1, *args[1:], tape=other, **kw)
+    stale = tr.trace_bwd(*common, tape=other, **bw)
+    torch.cuda.synchronize()
+    for i in (1, 4):
+        assert rel_l2(stale[i].cpu().numpy(), replay[i].cpu().numpy()) < 1e-5
+
+
+def test_autograd_function_gradients(small, device):
+    """Tracer._Autograd end to end: parameters requiring grad, an L2 loss on pred_pbr,
+    loss.backward(); the material and env gradients equal a direct trace_bwd call."""
+    from ptir_amd.tracer import LightBuffers, Tracer
+
+    sc, tab = small
+    tr = HipTracer(enable_metallic=True)
+    g = sc.to(device)
+    tr.build_bvh(g.positions, g.rotation, g.scale, g.density)
+    alb = g.albedo.clone().requires_grad_(True)
+    rgh = g.roughness.clone().requires_grad_(True)
+    met = g.metallic.clone().requires_grad_(True)
+    env = g.env.clone().requires_grad_(True)
+    outs = Tracer._Autograd.apply(tr, sc.frame, g.ray_to_world, g.rays_ori, g.rays_dir, g.positions, g.rotation,
+                                  g.scale, g.density, g.features, g.shading_normal, alb, rgh, met, env, tab.to(device),
+                                  LightBuffers.empty(device), 0, 3, sc.min_transmittance, sc.max_bounces)
+    target = scenes.target_image(sc).to(device)
+    loss = ((outs[10] - target) ** 2).mean()
+    loss.backward()
+    pbr_g = (2 * (outs[10].detach() - target) / outs[10].numel()).contiguous()
+    ref = gpu_backward(tr, sc, [o.detach() for o in tr.trace(sc.frame, g.ray_to_world, g.rays_ori, g.rays_dir,
+                                                              g.particle_density(), g.particle_material(), g.features,
+                                                              g.shading_normal, g.env, tab.to(device),
+                                                              max_bounces=sc.max_bounces)],
+                       (sc.frame, g.ray_to_world, g.rays_ori, g.rays_dir, g.particle_density(), g.particle_material(),
+                        g.features, g.shading_normal, g.env, tab.to(device)), pbr_g,
+                       torch.zeros_like(outs[11]), torch.zeros_like(outs[7]))
+    torch.cuda.synchronize()
+    got = torch.cat([alb.grad, rgh.grad, met.grad], 1)
+    assert rel_l2(got.cpu().numpy(), ref[1].cpu().numpy()) < 1e-5
+    assert rel_l2(env.grad.cpu().numpy(), ref[4].cpu().numpy()) < 1e-5
