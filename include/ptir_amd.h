@@ -107,7 +107,7 @@ typedef struct ptir_fwd_outputs {
      * No counterpart in the reference; pass the same buffer to ptir_trace_bwd. */
     void* tape;
     uint64_t tape_bytes;
-    uint32_t tape_hits_per_segment;
+    uint32_t tape_hits_per_path;     /* accepted-hit slots per path, shared by its segments */
 } ptir_fwd_outputs;
 
 /* Backward inputs: forward rayPbr/rayLight and the incoming gradients.  The reference's
@@ -124,7 +124,7 @@ typedef struct ptir_bwd_inputs {
      * ignored; paths whose hits overflowed the tape are re-traced. */
     const void* tape;
     uint64_t tape_bytes;
-    uint32_t tape_hits_per_segment;
+    uint32_t tape_hits_per_path;
 } ptir_bwd_inputs;
 
 /* Backward outputs (OptixTracer::traceBwd, optixTracer.cpp:1058-1063).  The library
@@ -170,10 +170,10 @@ ptir_status ptir_trace_bwd(ptir_tracer* tracer, const ptir_scene* scene, const p
                            void* stream);
 
 /* Bytes of a path tape for a [B,H,W] launch with max_bounces path vertices and room for
- * hits_per_segment accepted hits per traced segment (0 on invalid sizes).  No counterpart
- * in the reference: it lets ptir_trace_bwd skip re-tracing (a paths x 16 x 4 B record per
- * bounce plus 8 B per hit slot). */
-uint64_t ptir_tape_bytes(int32_t B, int32_t H, int32_t W, uint32_t max_bounces, uint32_t hits_per_segment);
+ * hits_per_path accepted hits per path (0 on invalid sizes).  No counterpart in the
+ * reference: it lets ptir_trace_bwd skip re-tracing (a 64 B record per path and per
+ * bounce, plus 8 B per hit slot). */
+uint64_t ptir_tape_bytes(int32_t B, int32_t H, int32_t W, uint32_t max_bounces, uint32_t hits_per_path);
 
 /* Human-readable text of the last error on this handle (or of the last ptir_create
  * failure when tracer is NULL).  Valid until the next call on the handle. */

@@ -51,12 +51,12 @@ class FwdOutputs(C.Structure):
     _fields_ = [(k, C.c_void_p) for k in ("radiance", "density", "hit_distance", "hit_distance2", "distortion",
                                           "normal", "shading_normal", "material", "hits_count", "visibility", "pbr",
                                           "light", "pbr_components", "tape")] + [
-        ("tape_bytes", C.c_uint64), ("tape_hits_per_segment", C.c_uint32)]
+        ("tape_bytes", C.c_uint64), ("tape_hits_per_path", C.c_uint32)]
 
 
 class BwdInputs(C.Structure):
     _fields_ = [(k, C.c_void_p) for k in ("pbr", "light", "pbr_grad", "light_grad", "material_grad", "tape")] + [
-        ("tape_bytes", C.c_uint64), ("tape_hits_per_segment", C.c_uint32)]
+        ("tape_bytes", C.c_uint64), ("tape_hits_per_path", C.c_uint32)]
 
 
 class BwdOutputs(C.Structure):

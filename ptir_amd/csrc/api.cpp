@@ -219,11 +219,11 @@ ptir_status ptir_trace(ptir_tracer* t, const ptir_scene* sc, const ptir_launch* 
                    out->shading_normal, out->material, out->hits_count, out->pbr, out->light, out->pbr_components,
                    reinterpret_cast<uint32_t*>(out->visibility), ptir::Tape{nullptr, 0u, 0u}};
     if (out->tape) {
-        const uint64_t need = ptir::tape_bytes((uint64_t)la->B * la->H * la->W, S.max_bounces, out->tape_hits_per_segment);
-        if (out->tape_hits_per_segment == 0 || out->tape_hits_per_segment > 0xFFFFu || out->tape_bytes < need)
+        const uint64_t need = ptir::tape_bytes((uint64_t)la->B * la->H * la->W, S.max_bounces, out->tape_hits_per_path);
+        if (out->tape_hits_per_path == 0 || out->tape_hits_per_path > 0xFFFFu || out->tape_bytes < need)
             return fail(t, PTIR_ERR_INVALID_ARGUMENT, "tape too small for this launch (" + std::to_string(out->tape_bytes) +
                                                          " < " + std::to_string(need) + " bytes)");
-        o.tape = ptir::Tape{static_cast<uint32_t*>(out->tape), out->tape_hits_per_segment, S.max_bounces};
+        o.tape = ptir::Tape{static_cast<uint32_t*>(out->tape), out->tape_hits_per_path, S.max_bounces};
     }
     e = ptir::launch_trace_fwd(S, o, t->opts.kbuffer, s);
     if (e != hipSuccess) return hip_fail(t, e, "trace kernel launch");
@@ -263,10 +263,10 @@ ptir_status ptir_trace_bwd(ptir_tracer* t, const ptir_scene* sc, const ptir_laun
     ptir::BwdIO io{in->pbr, in->light, in->pbr_grad, in->light_grad, in->material_grad, t->bvh.dmat_leaf,
                    sc->environment ? out->environment_grad : nullptr, ptir::Tape{nullptr, 0u, 0u}, t->bvh.tape_state, 0};
     if (in->tape) {
-        const uint64_t need = ptir::tape_bytes((uint64_t)la->B * la->H * la->W, S.max_bounces, in->tape_hits_per_segment);
-        if (in->tape_hits_per_segment == 0 || in->tape_hits_per_segment > 0xFFFFu || in->tape_bytes < need)
+        const uint64_t need = ptir::tape_bytes((uint64_t)la->B * la->H * la->W, S.max_bounces, in->tape_hits_per_path);
+        if (in->tape_hits_per_path == 0 || in->tape_hits_per_path > 0xFFFFu || in->tape_bytes < need)
             return fail(t, PTIR_ERR_INVALID_ARGUMENT, "tape too small for this launch");
-        io.tape = ptir::Tape{static_cast<uint32_t*>(const_cast<void*>(in->tape)), in->tape_hits_per_segment, S.max_bounces};
+        io.tape = ptir::Tape{static_cast<uint32_t*>(const_cast<void*>(in->tape)), in->tape_hits_per_path, S.max_bounces};
         e = ptir::fill_zero(t->bvh.tape_state, sizeof(int32_t), s);
         if (e != hipSuccess) return hip_fail(t, e, "zeroing the tape state");
     }
@@ -279,9 +279,9 @@ ptir_status ptir_trace_bwd(ptir_tracer* t, const ptir_scene* sc, const ptir_laun
     return end_counters(t, counters, s);
 }
 
-uint64_t ptir_tape_bytes(int32_t B, int32_t H, int32_t W, uint32_t max_bounces, uint32_t hits_per_segment) {
-    if (B < 0 || H < 0 || W < 0 || hits_per_segment == 0 || hits_per_segment > 0xFFFFu || max_bounces > 64u) return 0;
-    return ptir::tape_bytes((uint64_t)B * H * W, max_bounces, hits_per_segment);
+uint64_t ptir_tape_bytes(int32_t B, int32_t H, int32_t W, uint32_t max_bounces, uint32_t hits_per_path) {
+    if (B < 0 || H < 0 || W < 0 || hits_per_path == 0 || hits_per_path > 0xFFFFu || max_bounces > 64u) return 0;
+    return ptir::tape_bytes((uint64_t)B * H * W, max_bounces, hits_per_path);
 }
 
 const char* ptir_last_error(const ptir_tracer* t) { return t ? t->err.c_str() : g_create_error.c_str(); }

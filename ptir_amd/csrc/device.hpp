@@ -126,8 +126,8 @@ struct DeviceScene {
 // order, cut at the flush bound of pathTracer.cuh:326-329), the segment's transmittance,
 // flush bound, last hit and interaction, the NEE record and the RNG state after NEE.
 // Layout (32-bit words): header[kTapeHeaderWords], then per path a record of
-// kTapePathWords + kTapeSegWords * max_bounces words, then per path and segment
-// `cap` hit entries of 2 words.
+// kTapePathWords + kTapeSegWords * max_bounces words, then per path `cap` hit entries of
+// 2 words shared by the path's segments in order (a segment record keeps its offset).
 constexpr uint32_t kTapeMagic = 0x50544150u;  // "PTAP"
 constexpr uint32_t kTapeVersion = 1u;
 constexpr int kTapeHeaderWords = 64;
@@ -139,15 +139,15 @@ constexpr uint32_t kSegValid = 1u << 17;
 
 struct Tape {
     uint32_t* base;     // null: no tape
-    uint32_t cap;       // hit entries per segment
+    uint32_t cap;       // hit entries per path
     uint32_t segs;      // segments per path (= effective max bounces)
     __device__ __forceinline__ uint32_t rec_words() const { return kTapePathWords + kTapeSegWords * segs; }
     __device__ __forceinline__ uint32_t* path(size_t pix) const {
         return base + kTapeHeaderWords + pix * rec_words();
     }
-    __device__ __forceinline__ uint2* hits(size_t pix, size_t paths, uint32_t seg) const {
+    __device__ __forceinline__ uint2* hits(size_t pix, size_t paths) const {
         uint2* h = reinterpret_cast<uint2*>(base + kTapeHeaderWords + paths * rec_words());
-        return h + (pix * segs + seg) * cap;
+        return h + pix * cap;
     }
 };
 

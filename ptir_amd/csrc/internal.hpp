@@ -57,10 +57,10 @@ hipError_t launch_trace_bwd(const DeviceScene& S, const BwdIO& io, int kbuf, hip
 constexpr int kTraceBlock = 128;  // 2 waves: a 16x8 pixel tile
 constexpr int kStackDepth = 64;   // traversal stack entries per lane (LDS)
 
-// bytes of a tape with `segs` segments per path and `cap` hits per segment
+// bytes of a tape with `segs` segments per path and `cap` hits per path
 inline uint64_t tape_bytes(uint64_t paths, uint32_t segs, uint32_t cap) {
     const uint64_t rec = uint64_t(kTapePathWords) + uint64_t(kTapeSegWords) * segs;
-    return 4ull * (kTapeHeaderWords + paths * rec) + 8ull * paths * segs * cap;
+    return 4ull * (kTapeHeaderWords + paths * rec) + 8ull * paths * cap;
 }
 
 }  // namespace ptir

@@ -184,7 +184,7 @@ struct Path {  // payLoad.h:142-168
     float popacity, pmaxHit;
     V3 pdir;
     unsigned pbounces;
-    // tape segment index and hit count of the current and of the pending segment
+    // tape hit-list offset and hit count of the current and of the pending segment
     uint32_t cseg, ccnt, pseg, pcnt;
 };
 
@@ -561,6 +561,7 @@ struct SegTape {
     bool overflow;
     float tlast, tprev;  // distances of the last two recorded hits
     float endT;          // flush bound of the backward re-trace
+    uint32_t off;        // offset of this segment's first hit in the path's hit list
 };
 
 template <int K>
@@ -889,13 +890,13 @@ struct Flush {
     const Lane* L;
 };
 
-__device__ __forceinline__ void tape_flush(const DeviceScene& S, const Flush& f, uint32_t seg, uint32_t cnt, float opacity,
+__device__ __forceinline__ void tape_flush(const DeviceScene& S, const Flush& f, uint32_t off, uint32_t cnt, float opacity,
                                            V3 ga, float gr, float gm) {
     const float inv = 1.f / fmaxf(opacity, 1e-12f);
     ga = ga * inv;
     gr *= inv;
     gm *= inv;
-    const uint2* h = f.io->tape.hits(f.pix, f.paths, seg);
+    const uint2* h = f.io->tape.hits(f.pix, f.paths) + off;
     for (uint32_t i = 0; i < cnt; ++i) {
         const uint2 e = h[i];
         const float w = __uint_as_float(e.y);
@@ -1002,11 +1003,12 @@ __device__ __forceinline__ void store_seg(uint32_t* rec, uint32_t s, const Seg& 
     q[0] = make_float4(pl.T, tp.endT, pl.lastHit, __uint_as_float(packed));
     q[1] = make_float4(pl.pos.x, pl.pos.y, pl.pos.z, pl.sn.x);
     q[2] = make_float4(pl.sn.y, pl.sn.z, pl.alb.x, pl.alb.y);
-    q[3] = make_float4(pl.alb.z, pl.rough, pl.metal, 0.f);
+    q[3] = make_float4(pl.alb.z, pl.rough, pl.metal, __uint_as_float(tp.off));
 }
 
 // Segment s from the tape, traced along `ray`; keeps the MIS pdfs already set on pl.
-__device__ __forceinline__ uint32_t load_seg(const uint32_t* rec, uint32_t s, const Ray& ray, Seg& pl) {
+// Returns the hit count; `off` receives the offset of its hits in the path's hit list.
+__device__ __forceinline__ uint32_t load_seg(const uint32_t* rec, uint32_t s, const Ray& ray, Seg& pl, uint32_t& off) {
     const float4* q = reinterpret_cast<const float4*>(rec + kTapePathWords + kTapeSegWords * s);
     const float4 a = q[0], b = q[1], c = q[2], d = q[3];
     const float sPdf = pl.scatterPdf, lPdf = pl.lightPdf;
@@ -1022,12 +1024,15 @@ __device__ __forceinline__ uint32_t load_seg(const uint32_t* rec, uint32_t s, co
     pl.alb = v3(c.z, c.w, d.x);
     pl.rough = d.y;
     pl.metal = d.z;
+    off = __float_as_uint(d.w);
     return packed & 0xFFFFu;
 }
 
-__device__ __forceinline__ void seg_tape_begin(SegTape& tp, const Tape& t, size_t pix, size_t paths, uint32_t seg) {
-    tp.hits = t.hits(pix, paths, seg);
-    tp.cap = t.cap;
+// Start recording a segment after `used` entries of the path's hit list.
+__device__ __forceinline__ void seg_tape_begin(SegTape& tp, const Tape& t, size_t pix, size_t paths, uint32_t used) {
+    tp.hits = t.hits(pix, paths) + used;
+    tp.off = used;
+    tp.cap = t.cap - used;
     tp.n = 0;
     tp.overflow = false;
     tp.tlast = tp.tprev = -__int_as_float(0x7f800000);
@@ -1110,11 +1115,13 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_fwd(DeviceScene S, FwdOut
     Path p;
     path_init(p);
     count(S, C_PATHS);
-    if (rec) seg_tape_begin(tp, out.tape, pix, paths, 0);
+    uint32_t used = 0u;  // tape hit entries used by this path
+    if (rec) seg_tape_begin(tp, out.tape, pix, paths, used);
     ray_intersect<K>(S, primary_ray(S, pix), p.cur, rng, false, true, true, out.vis, L, tp);
     if (rec) {
         store_seg(prec, 0, p.cur, tp);
         flags |= tp.overflow ? kTapeOverflow : 0u;
+        used += tp.n;
     }
     {
         const Seg& c = p.cur;
@@ -1145,11 +1152,12 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_fwd(DeviceScene S, FwdOut
             p.numBounces = depth + 1u;
             sample_next(S, p, rng);
             if (vmax(p.thr) < 1e-4f) break;
-            if (rec) seg_tape_begin(tp, out.tape, pix, paths, p.numBounces);
+            if (rec) seg_tape_begin(tp, out.tape, pix, paths, used);
             ray_intersect<K>(S, p.cur.ray, p.cur, rng, true, false, true, out.vis, L, tp);
             if (rec) {
                 store_seg(prec, p.numBounces, p.cur, tp);
                 flags |= tp.overflow ? kTapeOverflow : 0u;
+                used += tp.n;
             }
         }
     }
@@ -1195,7 +1203,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_bwd(DeviceScene S, BwdIO 
     p.Lnb = v3(io.light[3 * pix], io.light[3 * pix + 1], io.light[3 * pix + 2]);
     if (io.light_grad) p.dLnb = v3(io.light_grad[3 * pix], io.light_grad[3 * pix + 1], io.light_grad[3 * pix + 2]);
     if constexpr (TAPE) {
-        p.ccnt = load_seg(prec, 0, primary_ray(S, pix), p.cur);
+        p.ccnt = load_seg(prec, 0, primary_ray(S, pix), p.cur, p.cseg);
         count(S, C_SEGMENTS);
     } else {
         SegTape none;
@@ -1232,8 +1240,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_bwd(DeviceScene S, BwdIO 
         sample_next_bwd<K, TAPE>(S, p, rng, f);
         if (vmax(p.thr) < 1e-4f) break;
         if constexpr (TAPE) {
-            p.cseg = p.numBounces;
-            p.ccnt = load_seg(prec, p.numBounces, p.cur.ray, p.cur);
+            p.ccnt = load_seg(prec, p.numBounces, p.cur.ray, p.cur, p.cseg);
             count(S, C_SEGMENTS);
             if (!p.cur.ivalid) (void)rng.next_1d();  // the forward's escape draw
         } else {

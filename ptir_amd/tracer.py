@@ -57,7 +57,7 @@ class HipTracer:
                  particle_kernel_density_clamping: bool = True, particle_radiance_sph_degree: int = 3,
                  enable_normals: bool = True, enable_hitcounts: bool = True, enable_mis: bool = True,
                  enable_metallic: bool = False, visualize_lights: bool = False, kbuffer: int = 16,
-                 use_tape: bool = True, tape_hits_per_segment: int = 32):
+                 use_tape: bool = True, tape_hits_per_path: int = 128):
         if pipeline != "reference" or backward_pipeline not in ("referenceBwd", "reference" + "Bwd"):
             raise ValueError(f"unsupported pipeline pair ({pipeline}, {backward_pipeline}); "
                              "only the PTIR 'reference' path tracer is built")
@@ -77,10 +77,10 @@ class HipTracer:
         # the autograd function records a path tape in the forward so that the backward
         # replays paths without re-tracing them (same gradients, no BVH traversal)
         self.use_tape = bool(use_tape)
-        self.tape_hits_per_segment = int(tape_hits_per_segment)
+        self.tape_hits_per_path = int(tape_hits_per_path)
 
     def tape_bytes(self, B: int, H: int, W: int, max_bounces: int) -> int:
-        return int(_lib.lib().ptir_tape_bytes(B, H, W, max_bounces, self.tape_hits_per_segment))
+        return int(_lib.lib().ptir_tape_bytes(B, H, W, max_bounces, self.tape_hits_per_path))
 
     def new_tape(self, ray_dir: torch.Tensor, max_bounces: int) -> torch.Tensor:
         """Device buffer for the path tape of one trace call on rays of ray_dir's shape."""
@@ -200,7 +200,7 @@ class HipTracer:
         out = _lib.FwdOutputs(*[_ptr(t) for t in (rad, dns, hit, hit2, dist, nrm, snrm, mat, hits, vis, pbr, light, comp)])
         if tape is not None:
             tape = _need(tape, "tape", dtype=torch.uint8)
-            out.tape, out.tape_bytes, out.tape_hits_per_segment = _ptr(tape), tape.numel(), self.tape_hits_per_segment
+            out.tape, out.tape_bytes, out.tape_hits_per_path = _ptr(tape), tape.numel(), self.tape_hits_per_path
         cnt = _lib.Counters() if counters else None
         st = _lib.lib().ptir_trace(self._h, C.byref(sc), C.byref(la), C.byref(out),
                                    C.byref(cnt) if cnt is not None else None, _stream())
@@ -251,7 +251,7 @@ class HipTracer:
         bi = _lib.BwdInputs(_ptr(pbr), _ptr(light), _ptr(pbr_g), _ptr(light_g), _ptr(mat_g))
         if tape is not None:
             tape = _need(tape, "tape", dtype=torch.uint8)
-            bi.tape, bi.tape_bytes, bi.tape_hits_per_segment = _ptr(tape), tape.numel(), self.tape_hits_per_segment
+            bi.tape, bi.tape_bytes, bi.tape_hits_per_path = _ptr(tape), tape.numel(), self.tape_hits_per_path
         bo = _lib.BwdOutputs(_ptr(d_dns), _ptr(d_mat), _ptr(d_sh), _ptr(d_sn), _ptr(d_env) if env is not None else None)
         if env is None and d_env.numel():
             d_env.zero_()

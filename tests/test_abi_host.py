@@ -41,7 +41,7 @@ def _opts(**kw):
 def test_tape_size_formula():
     L = _lib.lib()
     # header 64 words + per path (16 + 16 * segments) words + 8 bytes per hit slot
-    assert L.ptir_tape_bytes(2, 3, 4, 3, 32) == 4 * (64 + 24 * (16 + 48)) + 8 * 24 * 3 * 32
+    assert L.ptir_tape_bytes(2, 3, 4, 3, 32) == 4 * (64 + 24 * (16 + 48)) + 8 * 24 * 32
     assert L.ptir_tape_bytes(-1, 3, 4, 3, 32) == 0
     assert L.ptir_tape_bytes(1, 1, 1, 3, 0) == 0
 

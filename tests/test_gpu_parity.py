@@ -264,13 +264,13 @@ def test_graph_capture_replays_match_eager(small, device):
 
 
 # ----------------------------------------------------------------------------- path tape
-@pytest.mark.parametrize("cap", [32, 2])
+@pytest.mark.parametrize("cap", [128, 3])
 def test_tape_backward_equals_replay(small, device, cap):
     """The tape backward reads the forward's recorded hits instead of re-tracing; it must
-    give the replay's gradients (up to the order of float atomics).  cap=2 forces most
+    give the replay's gradients (up to the order of float atomics).  cap=3 forces most
     paths to overflow their tape and take the re-trace fallback."""
     sc, tab = small
-    tr = HipTracer(enable_metallic=True, tape_hits_per_segment=cap)
+    tr = HipTracer(enable_metallic=True, tape_hits_per_path=cap)
     g = sc.to(device)
     tr.build_bvh(g.positions, g.rotation, g.scale, g.density)
     args = (sc.frame, g.ray_to_world, g.rays_ori, g.rays_dir, g.particle_density(), g.particle_material(), g.features,
