@@ -142,6 +142,7 @@ typedef struct ptir_bwd_outputs {
 typedef struct ptir_counters {
     uint64_t paths, segments, candidates, accepted, accepted_primary, shadow_candidates, env_lookups;
     uint64_t mat_atomics, env_atomics;
+    uint64_t traversals, nodes, leaf_tests;  /* hierarchy work (no counterpart in the byte model) */
 } ptir_counters;
 
 /* OptixTracer::OptixTracer (optixTracer.cpp:274-343).  Allocates no device memory. */

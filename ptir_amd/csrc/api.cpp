@@ -138,6 +138,9 @@ ptir_status end_counters(ptir_tracer* t, ptir_counters* c, hipStream_t s) {
     c->env_lookups = h[ptir::C_ENV_LOOKUPS];
     c->mat_atomics = h[ptir::C_MAT_ATOMICS];
     c->env_atomics = h[ptir::C_ENV_ATOMICS];
+    c->traversals = h[ptir::C_TRAVERSALS];
+    c->nodes = h[ptir::C_NODES];
+    c->leaf_tests = h[ptir::C_LEAF_TESTS];
     if (flag) return fail(t, PTIR_ERR_BVH_DEPTH, "traversal stack overflow");
     return PTIR_OK;
 }

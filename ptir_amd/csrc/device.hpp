@@ -167,7 +167,7 @@ struct BwdIO {
 };
 
 enum CounterId { C_PATHS = 0, C_SEGMENTS, C_CANDIDATES, C_ACCEPTED, C_ACCEPTED_PRIMARY, C_SHADOW_CANDIDATES,
-                 C_ENV_LOOKUPS, C_MAT_ATOMICS, C_ENV_ATOMICS, C_COUNT };
+                 C_ENV_LOOKUPS, C_MAT_ATOMICS, C_ENV_ATOMICS, C_TRAVERSALS, C_NODES, C_LEAF_TESTS, C_COUNT };
 
 __device__ __forceinline__ void count(const DeviceScene& S, int id, unsigned long long v = 1) {
     if (S.counters) atomicAdd(S.counters + id, v);

@@ -84,8 +84,10 @@ __device__ __forceinline__ int collect(const DeviceScene& S, const Ray& r, float
     };
     auto leaf = [&](int slot) {
         float t;
+        count(S, C_LEAF_TESTS);
         if (cand_test(S, slot, r, tmin, tmax, t)) insert(t, uint32_t(slot));
     };
+    count(S, C_TRAVERSALS);
     const V3 inv = v3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
     if (S.n > 0) {
         int link = S.root;
@@ -96,6 +98,7 @@ __device__ __forceinline__ int collect(const DeviceScene& S, const Ray& r, float
             for (;;) {
                 const float4* nd = S.nodes + 4 * link;
                 const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2], n3 = nd[3];
+                count(S, C_NODES);
                 const float cut = fminf(tmax, bt[K - 1]);
                 float tl, tr;
                 bool hl = slab(n0, n1, r, inv, tmin, cut, tl);
