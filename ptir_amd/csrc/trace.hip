@@ -89,40 +89,70 @@ __device__ __forceinline__ int collect(const DeviceScene& S, const Ray& r, float
     };
     count(S, C_TRAVERSALS);
     const V3 inv = v3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
-    if (S.n > 0) {
+    if (S.n > 0 && S.root < 0) {
+        leaf(~S.root);
+    } else if (S.n > 0) {
+        // Node steps and leaf tests run in separate phases so that each phase keeps most
+        // lanes of the wave busy.  A lane queues the leaves it reaches (up to four) and
+        // keeps walking while it has room; the node phase ends once every lane still
+        // walking has a leaf queued, then all queues drain together.  The K nearest
+        // candidates do not depend on the order of the tests; culling against the
+        // K-th distance only gets later, never wrong.
         int link = S.root;
-        if (link < 0) {
-            leaf(~link);
-        } else {
-            int sp = 0;
+        int sp = 0;
+        bool walk = true;             // nodes left to visit
+        int q0 = -1, q1 = -1, q2 = -1, q3 = -1;  // queued leaf slots, q0 first
+        auto push = [&](int s) {
+            q3 = q2;
+            q2 = q1;
+            q1 = q0;
+            q0 = s;
+        };
+        for (;;) {
             for (;;) {
-                const float4* nd = S.nodes + 4 * link;
-                const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2], n3 = nd[3];
-                count(S, C_NODES);
-                const float cut = fminf(tmax, bt[K - 1]);
-                float tl, tr;
-                bool hl = slab(n0, n1, r, inv, tmin, cut, tl);
-                bool hr = slab(n2, n3, r, inv, tmin, cut, tr);
-                const int cl = __float_as_int(n0.w), cr = __float_as_int(n1.w);
-                if (hl && cl < 0) { leaf(~cl); hl = false; }
-                if (hr && cr < 0) { leaf(~cr); hr = false; }
-                if (hl && hr) {
-                    const bool lf = tl <= tr;
-                    if (sp >= kStackDepth) {  // cannot happen after the build-time depth check
-                        atomicOr(S.error_flag, 1);
-                        break;
+                if (__ballot(walk && q0 < 0) == 0) break;  // every walking lane has a leaf queued
+                if (walk && q2 < 0) {                         // room for both children
+                    const float4* nd = S.nodes + 4 * link;
+                    const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2], n3 = nd[3];
+                    count(S, C_NODES);
+                    const float cut = fminf(tmax, bt[K - 1]);
+                    float tl, tr;
+                    bool hl = slab(n0, n1, r, inv, tmin, cut, tl);
+                    bool hr = slab(n2, n3, r, inv, tmin, cut, tr);
+                    const int cl = __float_as_int(n0.w), cr = __float_as_int(n1.w);
+                    if (hl && cl < 0) { push(~cl); hl = false; }
+                    if (hr && cr < 0) { push(~cr); hr = false; }
+                    if (hl && hr) {
+                        const bool lf = tl <= tr;
+                        if (sp >= kStackDepth) {  // cannot happen after the build-time depth check
+                            atomicOr(S.error_flag, 1);
+                            walk = false;
+                        } else {
+                            L.at(sp++) = uint32_t(lf ? cr : cl);
+                            link = lf ? cl : cr;
+                        }
+                    } else if (hl) {
+                        link = cl;
+                    } else if (hr) {
+                        link = cr;
+                    } else if (sp == 0) {
+                        walk = false;
+                    } else {
+                        link = int(L.at(--sp));
                     }
-                    L.at(sp++) = uint32_t(lf ? cr : cl);
-                    link = lf ? cl : cr;
-                } else if (hl) {
-                    link = cl;
-                } else if (hr) {
-                    link = cr;
-                } else {
-                    if (sp == 0) break;
-                    link = int(L.at(--sp));
                 }
             }
+            while (__ballot(q0 >= 0) != 0) {
+                if (q0 >= 0) {
+                    const int slot = q0;
+                    q0 = q1;
+                    q1 = q2;
+                    q2 = q3;
+                    q3 = -1;
+                    leaf(slot);
+                }
+            }
+            if (__ballot(walk) == 0) break;
         }
     }
     (void)ncand;
@@ -567,9 +597,132 @@ struct SegTape {
     uint32_t off;        // offset of this segment's first hit in the path's hit list
 };
 
+// Wave-coherent front-to-back collection for the rays of one wave (the 8x8 primary tile).
+// The wave walks the hierarchy once with a wave-uniform node stack (node and record
+// loads are uniform); every lane tests the children's boxes with its own ray and keeps
+// its own K-buffer.  A lane may composite its nearest pending candidate as soon as its
+// t is below the lane's safe distance: the smallest box-entry distance of any node still
+// on the stack (kept per lane as a prefix minimum in LDS), because a candidate's peak
+// lies inside its box.  So hits are emitted in exactly increasing (t, slot) order.  A
+// lane whose K-buffer would overflow stops and reports it; the caller finishes that
+// segment with the per-lane chunked traversal from the last emitted hit.
+template <int K, class Proc, class Alive>
+__device__ __forceinline__ bool tile_collect(const DeviceScene& S, const Ray& r, float tmin, float tmax, bool need,
+                                             const Lane& L, uint32_t* wstack, Proc&& proc, Alive&& alive) {
+    const float kInf = __int_as_float(0x7f800000);
+    float bt[K];
+    uint32_t bs[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        bt[j] = kInf;
+        bs[j] = kInvalid;
+    }
+    int cnt = 0;
+    bool ovf = false;
+    const V3 inv = v3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
+    auto insert = [&](float t, uint32_t s) {
+        if (cnt == K) {  // a candidate would be dropped: hand the segment to the chunked traversal
+            ovf = true;
+            return;
+        }
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const bool lt = (t < bt[j]) || (t == bt[j] && s < bs[j]);
+            const float tt = lt ? bt[j] : t;
+            const uint32_t ss = lt ? bs[j] : s;
+            bt[j] = lt ? t : bt[j];
+            bs[j] = lt ? s : bs[j];
+            t = tt;
+            s = ss;
+        }
+        ++cnt;
+    };
+    auto leaf = [&](int slot) {
+        float t;
+        count(S, C_LEAF_TESTS);
+        if (cand_test(S, slot, r, tmin, tmax, t)) insert(t, uint32_t(slot));
+    };
+    auto emit = [&](float safe) {
+        while (cnt > 0 && bt[0] < safe && alive()) {
+            proc(bt[0], int(bs[0]));
+#pragma unroll
+            for (int j = 0; j < K - 1; ++j) {
+                bt[j] = bt[j + 1];
+                bs[j] = bs[j + 1];
+            }
+            bt[K - 1] = kInf;
+            bs[K - 1] = kInvalid;
+            --cnt;
+        }
+    };
+    count(S, C_TRAVERSALS);
+    if (S.n == 0) need = false;
+    if (S.root < 0) {
+        if (need) leaf(~S.root);
+        if (need && !ovf) emit(kInf);
+        return ovf;
+    }
+    int link = S.root;
+    int sp = 0;  // wave-uniform
+    for (;;) {
+        need = need && !ovf && alive();
+        if (__ballot(need) == 0) break;
+        link = __builtin_amdgcn_readfirstlane(link);
+        const float4* nd = S.nodes + 4 * link;
+        const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2], n3 = nd[3];
+        count(S, C_NODES);
+        float tl = kInf, tr = kInf;
+        const bool hl = need && slab(n0, n1, r, inv, tmin, tmax, tl);
+        const bool hr = need && slab(n2, n3, r, inv, tmin, tmax, tr);
+        const int cl = __builtin_amdgcn_readfirstlane(__float_as_int(n0.w));
+        const int cr = __builtin_amdgcn_readfirstlane(__float_as_int(n1.w));
+        const uint64_t bl = __ballot(hl), br = __ballot(hr);
+        if (cl < 0 && bl != 0 && hl) leaf(~cl);
+        if (cr < 0 && br != 0 && hr) leaf(~cr);
+        const bool gl = cl >= 0 && bl != 0, gr = cr >= 0 && br != 0;
+        const float tnl = hl ? tl : kInf, tnr = hr ? tr : kInf;
+        const float pm = sp > 0 ? __uint_as_float(L.at(sp - 1)) : kInf;  // min entry over the stack
+        float safe;
+        if (gl && gr) {
+            // near child first, as seen by the first lane that hits both (or either)
+            uint64_t both = __ballot(hl && hr);
+            if (both == 0) both = bl | br;
+            const int leader = __ffsll((long long)both) - 1;
+            const bool lf = __shfl(tnl <= tnr ? 1 : 0, leader) != 0;
+            if (sp >= kStackDepth) {  // cannot happen after the build-time depth check
+                atomicOr(S.error_flag, 1);
+                break;
+            }
+            const float far_tn = lf ? tnr : tnl;
+            L.at(sp) = __float_as_uint(fminf(pm, far_tn));
+            wstack[sp] = uint32_t(lf ? cr : cl);  // every active lane stores the same uniform link
+            ++sp;
+            link = lf ? cl : cr;
+            safe = fminf(fminf(pm, far_tn), lf ? tnl : tnr);
+        } else if (gl) {
+            link = cl;
+            safe = fminf(pm, tnl);
+        } else if (gr) {
+            link = cr;
+            safe = fminf(pm, tnr);
+        } else if (sp > 0) {
+            --sp;
+            link = int(wstack[sp]);
+            safe = pm;  // includes the popped node's own entry distance
+        } else {
+            link = -1;
+            safe = kInf;
+        }
+        if (need && !ovf) emit(safe);
+        if (link < 0) break;
+    }
+    if (need && !ovf) emit(kInf);
+    return ovf;
+}
+
 template <int K>
 __device__ __forceinline__ void ray_intersect(const DeviceScene& S, const Ray ray, Seg& pl, Rng& rng, bool selfOcc, bool primary,
-                              bool gbuffer, uint32_t* vis, const Lane& L, SegTape& tp) {
+                              bool gbuffer, uint32_t* vis, const Lane& L, SegTape& tp, uint32_t* wstack = nullptr) {
     float t0, t1;
     scene_interval(S, ray, t0, t1);
     const float sPdf = pl.scatterPdf, lPdf = pl.lightPdf;
@@ -586,60 +739,74 @@ __device__ __forceinline__ void ray_intersect(const DeviceScene& S, const Ray ra
     const bool need_nrm = primary && gbuffer && S.enable_normals;
     const float dlen = sqrtf(dot(ray.d, ray.d));
     count(S, C_SEGMENTS);
-    while (pl.lastHit <= t1 && pl.T > minT) {
-        const int cnt = collect<K>(S, ray, pl.lastHit + kEpsT, t1 + kEpsT, L);
-        if (cnt == 0) break;
-        for (int k = 0; k < cnt; ++k) {
-            if (!(pl.T > minT)) break;
-            const float t = __uint_as_float(L.at(k));
-            const int slot = int(L.at(K + k));
-            count(S, C_CANDIDATES);
-            if (selfOcc && self_occluded(S, slot, t, ray.d)) {
-                pl.lastHit = fmaxf(pl.lastHit, t);
-                continue;
-            }
-            HitGeom h;
-            const float a = hit_alpha(S, slot, ray, h);
-            const bool acc = a > 0.f;
-            if (acc) {
-                const float w = a * pl.T;
-                const float hitT = t * dlen;
-                const uint32_t id = orig_id(S, slot);
-                if (need_rad) pl.radiance += sh_radiance(S, id, ray.d) * w;
-                const float4 m0 = S.srec[2 * slot], m1 = S.srec[2 * slot + 1];
-                alb += xyz(m0) * w;
-                rough += m0.w * w;
-                metal += m1.x * w;
-                pl.T *= (1.f - a);
-                depth += hitT * w;
-                pl.hitDist2 += hitT * hitT * w;
-                if (S.enable_normals) {
-                    snAcc += w * v3(m1.y, m1.z, m1.w);
-                    if (need_nrm) {
-                        const float4 rs = S.prec[slot];
-                        const float proj = dot(h.grd, -h.gro);
-                        pl.normal += w * safe_normalize((h.gro + h.grd * (proj - sqrtf(9.f - h.gd))) * xyz(rs));
-                    }
-                }
-                if (tp.hits) {
-                    if (tp.n < tp.cap) {
-                        tp.hits[tp.n] = make_uint2(uint32_t(slot), __float_as_uint(w));
-                        ++tp.n;
-                        tp.tprev = tp.tlast;
-                        tp.tlast = t;
-                    } else {
-                        tp.overflow = true;
-                    }
-                }
-                pl.hit = true;
-                const float op = 1.f - pl.T;
-                pl.distortion = fmaxf(op * pl.hitDist2 - depth * depth, 0.f);
-                if (vis) vis[id] = 1u;
-                count(S, C_ACCEPTED);
-                if (primary) count(S, C_ACCEPTED_PRIMARY);
-            }
+    // gaussianParticles.cuh:343-440 processHit and pathTracer.cuh:245-292 for one candidate
+    auto proc = [&](float t, int slot) {
+        count(S, C_CANDIDATES);
+        if (selfOcc && self_occluded(S, slot, t, ray.d)) {
             pl.lastHit = fmaxf(pl.lastHit, t);
-            if (S.enable_hitcounts) pl.hits += acc ? 1.f : 0.f;
+            return;
+        }
+        HitGeom h;
+        const float a = hit_alpha(S, slot, ray, h);
+        const bool acc = a > 0.f;
+        if (acc) {
+            const float w = a * pl.T;
+            const float hitT = t * dlen;
+            const uint32_t id = orig_id(S, slot);
+            if (need_rad) pl.radiance += sh_radiance(S, id, ray.d) * w;
+            const float4 m0 = S.srec[2 * slot], m1 = S.srec[2 * slot + 1];
+            alb += xyz(m0) * w;
+            rough += m0.w * w;
+            metal += m1.x * w;
+            pl.T *= (1.f - a);
+            depth += hitT * w;
+            pl.hitDist2 += hitT * hitT * w;
+            if (S.enable_normals) {
+                snAcc += w * v3(m1.y, m1.z, m1.w);
+                if (need_nrm) {
+                    const float4 rs = S.prec[slot];
+                    const float proj = dot(h.grd, -h.gro);
+                    pl.normal += w * safe_normalize((h.gro + h.grd * (proj - sqrtf(9.f - h.gd))) * xyz(rs));
+                }
+            }
+            if (tp.hits) {
+                if (tp.n < tp.cap) {
+                    tp.hits[tp.n] = make_uint2(uint32_t(slot), __float_as_uint(w));
+                    ++tp.n;
+                    tp.tprev = tp.tlast;
+                    tp.tlast = t;
+                } else {
+                    tp.overflow = true;
+                }
+            }
+            pl.hit = true;
+            const float op = 1.f - pl.T;
+            pl.distortion = fmaxf(op * pl.hitDist2 - depth * depth, 0.f);
+            // the flag is idempotent: skip the store when a previous path already set it
+            if (vis && vis[id] == 0u) vis[id] = 1u;
+            count(S, C_ACCEPTED);
+            if (primary) count(S, C_ACCEPTED_PRIMARY);
+        }
+        pl.lastHit = fmaxf(pl.lastHit, t);
+        if (S.enable_hitcounts) pl.hits += acc ? 1.f : 0.f;
+    };
+    bool chunked = true;
+    if (wstack) {
+        // all lanes of the wave arrive here together (the primary segment)
+        const bool need = pl.lastHit <= t1 && pl.T > minT;
+        const bool ovf = tile_collect<K>(S, ray, pl.lastHit + kEpsT, t1 + kEpsT, need, L, wstack, proc,
+                                         [&]() { return pl.T > minT; });
+        chunked = ovf;
+        if (ovf) count(S, C_TILE_FALLBACKS);
+    }
+    if (chunked) {
+        while (pl.lastHit <= t1 && pl.T > minT) {
+            const int cnt = collect<K>(S, ray, pl.lastHit + kEpsT, t1 + kEpsT, L);
+            if (cnt == 0) break;
+            for (int k = 0; k < cnt; ++k) {
+                if (!(pl.T > minT)) break;
+                proc(__uint_as_float(L.at(k)), int(L.at(K + k)));
+            }
         }
     }
     if (tp.hits) {
@@ -1098,7 +1265,9 @@ __device__ __forceinline__ void path_init(Path& p) {
 template <int K>
 __global__ void __launch_bounds__(kTraceBlock) k_trace_fwd(DeviceScene S, FwdOut out) {
     __shared__ uint32_t lds[LdsLayout<K>::kWords * kTraceBlock];
+    __shared__ uint32_t wst[(kTraceBlock / 64) * kStackDepth];  // wave-uniform node stacks
     const Lane L{lds + threadIdx.x};
+    uint32_t* wstack = wst + (threadIdx.x / 64) * kStackDepth;
     const PixelId px = pixel_of(S);
     if (!px.ok) return;
     const size_t pix = (size_t(px.z) * S.H + px.y) * S.W + px.x;
@@ -1120,7 +1289,8 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_fwd(DeviceScene S, FwdOut
     count(S, C_PATHS);
     uint32_t used = 0u;  // tape hit entries used by this path
     if (rec) seg_tape_begin(tp, out.tape, pix, paths, used);
-    ray_intersect<K>(S, primary_ray(S, pix), p.cur, rng, false, true, true, out.vis, L, tp);
+    ray_intersect<K>(S, primary_ray(S, pix), p.cur, rng, false, true, true, out.vis, L, tp,
+                     S.tile_primary ? wstack : nullptr);
     if (rec) {
         store_seg(prec, 0, p.cur, tp);
         flags |= tp.overflow ? kTapeOverflow : 0u;

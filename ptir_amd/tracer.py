@@ -57,7 +57,7 @@ class HipTracer:
                  particle_kernel_density_clamping: bool = True, particle_radiance_sph_degree: int = 3,
                  enable_normals: bool = True, enable_hitcounts: bool = True, enable_mis: bool = True,
                  enable_metallic: bool = False, visualize_lights: bool = False, kbuffer: int = 16,
-                 use_tape: bool = True, tape_hits_per_path: int = 128):
+                 use_tape: bool = True, tape_hits_per_path: int = 128, primary_traversal: str = "lane"):
         if pipeline != "reference" or backward_pipeline not in ("referenceBwd", "reference" + "Bwd"):
             raise ValueError(f"unsupported pipeline pair ({pipeline}, {backward_pipeline}); "
                              "only the PTIR 'reference' path tracer is built")
@@ -66,7 +66,8 @@ class HipTracer:
         o = _lib.Options(float(particle_kernel_degree), float(particle_kernel_min_response),
                          int(bool(particle_kernel_density_clamping)), int(particle_radiance_sph_degree),
                          int(bool(enable_normals)), int(bool(enable_hitcounts)), int(bool(enable_mis)),
-                         int(bool(enable_metallic)), int(bool(visualize_lights)), int(kbuffer))
+                         int(bool(enable_metallic)), int(bool(visualize_lights)), int(kbuffer),
+                         {"lane": 0, "tile": 1}[primary_traversal])
         h = C.c_void_p()
         L = _lib.lib()
         _lib.check(L.ptir_create(C.byref(o), C.byref(h)), None)
