@@ -227,10 +227,14 @@ ptir_status ptir_trace(ptir_tracer* t, const ptir_scene* sc, const ptir_launch* 
                    reinterpret_cast<uint32_t*>(out->visibility), ptir::Tape{nullptr, 0u, 0u}};
     if (out->tape) {
         const uint64_t need = ptir::tape_bytes((uint64_t)la->B * la->H * la->W, S.max_bounces, out->tape_hits_per_path);
-        if (out->tape_hits_per_path == 0 || out->tape_hits_per_path > 0xFFFFu || out->tape_bytes < need)
+        if (out->tape_hits_per_path == 0 || out->tape_hits_per_path > 0xFFFFu - ptir::kTapeChunk ||
+            out->tape_bytes < need)
             return fail(t, PTIR_ERR_INVALID_ARGUMENT, "tape too small for this launch (" + std::to_string(out->tape_bytes) +
                                                          " < " + std::to_string(need) + " bytes)");
         o.tape = ptir::Tape{static_cast<uint32_t*>(out->tape), out->tape_hits_per_path, S.max_bounces};
+        // the pool's chunk counter starts at zero (a kernel, so the call stays capturable)
+        e = ptir::fill_zero(o.tape.base + ptir::kTapePoolNext, sizeof(uint32_t), s);
+        if (e != hipSuccess) return hip_fail(t, e, "zeroing the tape pool counter");
     }
     e = ptir::launch_trace_fwd(S, o, t->opts.kbuffer, s);
     if (e != hipSuccess) return hip_fail(t, e, "trace kernel launch");
@@ -271,7 +275,7 @@ ptir_status ptir_trace_bwd(ptir_tracer* t, const ptir_scene* sc, const ptir_laun
                    sc->environment ? out->environment_grad : nullptr, ptir::Tape{nullptr, 0u, 0u}, t->bvh.tape_state, 0};
     if (in->tape) {
         const uint64_t need = ptir::tape_bytes((uint64_t)la->B * la->H * la->W, S.max_bounces, in->tape_hits_per_path);
-        if (in->tape_hits_per_path == 0 || in->tape_hits_per_path > 0xFFFFu || in->tape_bytes < need)
+        if (in->tape_hits_per_path == 0 || in->tape_hits_per_path > 0xFFFFu - ptir::kTapeChunk || in->tape_bytes < need)
             return fail(t, PTIR_ERR_INVALID_ARGUMENT, "tape too small for this launch");
         io.tape = ptir::Tape{static_cast<uint32_t*>(const_cast<void*>(in->tape)), in->tape_hits_per_path, S.max_bounces};
         e = ptir::fill_zero(t->bvh.tape_state, sizeof(int32_t), s);

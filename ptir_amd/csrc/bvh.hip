@@ -302,7 +302,7 @@ hipError_t bvh_reserve(BvhBuffers& b, uint32_t n) {
     e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, b.keys, b.keys_sorted, b.idx, b.leaf_id, int(n), 0, 63);
     if (e != hipSuccess) return e;
     if (tmp > b.sort_tmp_bytes) {
-        if (b.sort_tmp) hipFree(b.sort_tmp);
+        if (b.sort_tmp) (void)hipFree(b.sort_tmp);
         b.sort_tmp = nullptr;
         if ((e = hipMalloc(&b.sort_tmp, tmp)) != hipSuccess) return e;
         b.sort_tmp_bytes = tmp;

@@ -128,19 +128,30 @@ struct DeviceScene {
 // flush bound, last hit and interaction, the NEE record and the RNG state after NEE.
 // Layout (32-bit words): header[kTapeHeaderWords], then per path a record of
 // kTapePathWords + kTapeSegWords * max_bounces words, then per path `cap` hit entries of
-// 2 words shared by the path's segments in order (a segment record keeps its offset).
+// 2 words shared by the path's segments in order (a segment record keeps its offset),
+// then a pool of `pool_chunks` chunks of kTapeChunk entries: a path whose hits outgrow its
+// own `cap` entries takes one chunk (atomic counter in header word kTapePoolNext) and
+// continues there.  Entry i of a path is own[i] for i < cap, else chunk[i - cap].
 constexpr uint32_t kTapeMagic = 0x50544150u;  // "PTAP"
-constexpr uint32_t kTapeVersion = 1u;
+constexpr uint32_t kTapeVersion = 2u;
 constexpr int kTapeHeaderWords = 64;
+constexpr int kTapePoolNext = 16;        // header word: chunks taken so far (zeroed before the forward)
 constexpr int kTapePathWords = 16;
 constexpr int kTapeSegWords = 16;
+constexpr uint32_t kTapeChunk = 128;     // hit entries per pool chunk
+constexpr uint32_t kTapePathsPerChunk = 64;
+constexpr uint32_t kNoChunk = 0xffffffffu;
 constexpr uint32_t kTapeOverflow = 1u;   // path flag: replay this path by re-tracing
 constexpr uint32_t kSegIValid = 1u << 16;
 constexpr uint32_t kSegValid = 1u << 17;
 
+__host__ __device__ inline uint32_t tape_pool_chunks(uint64_t paths) {
+    return uint32_t((paths + kTapePathsPerChunk - 1) / kTapePathsPerChunk);
+}
+
 struct Tape {
     uint32_t* base;     // null: no tape
-    uint32_t cap;       // hit entries per path
+    uint32_t cap;       // own hit entries per path
     uint32_t segs;      // segments per path (= effective max bounces)
     __device__ __forceinline__ uint32_t rec_words() const { return kTapePathWords + kTapeSegWords * segs; }
     __device__ __forceinline__ uint32_t* path(size_t pix) const {
@@ -149,6 +160,13 @@ struct Tape {
     __device__ __forceinline__ uint2* hits(size_t pix, size_t paths) const {
         uint2* h = reinterpret_cast<uint2*>(base + kTapeHeaderWords + paths * rec_words());
         return h + pix * cap;
+    }
+    __device__ __forceinline__ uint2* pool(size_t paths) const {
+        return reinterpret_cast<uint2*>(base + kTapeHeaderWords + paths * rec_words()) + paths * cap;
+    }
+    // entry i of a path whose own entries start at `own` (chunk: its pool chunk or kNoChunk)
+    __device__ __forceinline__ uint2* entry(uint2* own, uint2* pl, uint32_t chunk, uint32_t i) const {
+        return i < cap ? own + i : pl + size_t(chunk) * kTapeChunk + (i - cap);
     }
 };
 

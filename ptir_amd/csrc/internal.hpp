@@ -60,7 +60,7 @@ constexpr int kStackDepth = 64;   // traversal stack entries per lane (LDS)
 // bytes of a tape with `segs` segments per path and `cap` hits per path
 inline uint64_t tape_bytes(uint64_t paths, uint32_t segs, uint32_t cap) {
     const uint64_t rec = uint64_t(kTapePathWords) + uint64_t(kTapeSegWords) * segs;
-    return 4ull * (kTapeHeaderWords + paths * rec) + 8ull * paths * cap;
+    return 4ull * (kTapeHeaderWords + paths * rec) + 8ull * paths * cap + 8ull * kTapeChunk * tape_pool_chunks(paths);
 }
 
 }  // namespace ptir

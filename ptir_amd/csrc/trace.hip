@@ -587,14 +587,45 @@ __device__ __forceinline__ V3 sh_radiance(const DeviceScene& S, uint32_t id, V3 
 }
 
 // pathTracer.cuh:216-311 rayIntersect with gaussianParticles.cuh:343-440 processHit
-// Tape writer of one segment: accepted hits in processing order.
+// Tape writer of one segment: accepted hits in processing order, appended to the path's
+// hit list (own entries, then a pool chunk taken on first need).
 struct SegTape {
-    uint2* hits;     // null: do not record
-    uint32_t cap, n;
+    uint2* hits;         // the path's own entries; null: do not record
+    uint2* pool;
+    uint32_t* pool_next;
+    uint32_t pool_chunks;
+    uint32_t cap;        // own entries per path
+    uint32_t chunk;      // the path's pool chunk (kNoChunk: none yet)
+    uint32_t off;        // index of this segment's first hit in the path's hit list
+    uint32_t n;
     bool overflow;
-    float tlast, tprev;  // distances of the last two recorded hits
+    float tlast;         // distance of the last recorded hit
+    uint32_t ties;       // recorded hits at exactly tlast (the tail the flush bound may cut)
+    float tbefore;       // distance of the last recorded hit before that tail
     float endT;          // flush bound of the backward re-trace
-    uint32_t off;        // offset of this segment's first hit in the path's hit list
+    __device__ __forceinline__ void record(uint32_t slot, float w, float t) {
+        const uint32_t i = off + n;
+        uint2* dst = nullptr;
+        if (i < cap) {
+            dst = hits + i;
+        } else if (i - cap < kTapeChunk) {
+            if (chunk == kNoChunk) chunk = atomicAdd(pool_next, 1u);
+            if (chunk < pool_chunks) dst = pool + size_t(chunk) * kTapeChunk + (i - cap);
+        }
+        if (!dst) {
+            overflow = true;
+            return;
+        }
+        *dst = make_uint2(slot, __float_as_uint(w));
+        ++n;
+        if (t == tlast) {
+            ++ties;
+        } else {
+            tbefore = tlast;
+            tlast = t;
+            ties = 1u;
+        }
+    }
 };
 
 // Wave-coherent front-to-back collection for the rays of one wave (the 8x8 primary tile).
@@ -769,16 +800,7 @@ __device__ __forceinline__ void ray_intersect(const DeviceScene& S, const Ray ra
                     pl.normal += w * safe_normalize((h.gro + h.grd * (proj - sqrtf(9.f - h.gd))) * xyz(rs));
                 }
             }
-            if (tp.hits) {
-                if (tp.n < tp.cap) {
-                    tp.hits[tp.n] = make_uint2(uint32_t(slot), __float_as_uint(w));
-                    ++tp.n;
-                    tp.tprev = tp.tlast;
-                    tp.tlast = t;
-                } else {
-                    tp.overflow = true;
-                }
-            }
+            if (tp.hits) tp.record(uint32_t(slot), w, t);
             pl.hit = true;
             const float op = 1.f - pl.T;
             pl.distortion = fmaxf(op * pl.hitDist2 - depth * depth, 0.f);
@@ -814,8 +836,8 @@ __device__ __forceinline__ void ray_intersect(const DeviceScene& S, const Ray ra
         // in float32 the bound usually equals lastHit, which drops the last processed hit
         tp.endT = fminf(pl.lastHit, t1) + kEpsT;
         if (tp.n > 0 && tp.tlast >= tp.endT) {
-            --tp.n;
-            if (tp.n > 0 && tp.tprev >= tp.endT) tp.overflow = true;  // 3-way tie: the replay handles it
+            tp.n -= tp.ties;  // every hit at the last distance falls outside the bound
+            if (tp.n > 0 && tp.tbefore >= tp.endT) tp.overflow = true;  // cannot happen; the replay would handle it
         }
     }
     const float op = 1.f - pl.T;
@@ -1058,6 +1080,7 @@ struct Flush {
     const BwdIO* io;
     size_t pix, paths;
     const Lane* L;
+    uint32_t chunk;  // the path's tape pool chunk (kNoChunk: none)
 };
 
 __device__ __forceinline__ void tape_flush(const DeviceScene& S, const Flush& f, uint32_t off, uint32_t cnt, float opacity,
@@ -1066,9 +1089,11 @@ __device__ __forceinline__ void tape_flush(const DeviceScene& S, const Flush& f,
     ga = ga * inv;
     gr *= inv;
     gm *= inv;
-    const uint2* h = f.io->tape.hits(f.pix, f.paths) + off;
-    for (uint32_t i = 0; i < cnt; ++i) {
-        const uint2 e = h[i];
+    const Tape& tape = f.io->tape;
+    uint2* own = tape.hits(f.pix, f.paths);
+    uint2* pl = tape.pool(f.paths);
+    for (uint32_t i = off; i < off + cnt; ++i) {
+        const uint2 e = *tape.entry(own, pl, f.chunk, i);
         const float w = __uint_as_float(e.y);
         float* q = f.io->dmat_leaf + 5 * size_t(e.x);
         atomicAdd(q + 0, w * ga.x);
@@ -1198,14 +1223,23 @@ __device__ __forceinline__ uint32_t load_seg(const uint32_t* rec, uint32_t s, co
     return packed & 0xFFFFu;
 }
 
+// Start recording a path: its own hit entries, no pool chunk yet.
+__device__ __forceinline__ void path_tape_begin(SegTape& tp, const Tape& t, size_t pix, size_t paths) {
+    tp.hits = t.hits(pix, paths);
+    tp.pool = t.pool(paths);
+    tp.pool_next = t.base + kTapePoolNext;
+    tp.pool_chunks = tape_pool_chunks(paths);
+    tp.cap = t.cap;
+    tp.chunk = kNoChunk;
+}
+
 // Start recording a segment after `used` entries of the path's hit list.
-__device__ __forceinline__ void seg_tape_begin(SegTape& tp, const Tape& t, size_t pix, size_t paths, uint32_t used) {
-    tp.hits = t.hits(pix, paths) + used;
+__device__ __forceinline__ void seg_tape_begin(SegTape& tp, uint32_t used) {
     tp.off = used;
-    tp.cap = t.cap - used;
     tp.n = 0;
     tp.overflow = false;
-    tp.tlast = tp.tprev = -__int_as_float(0x7f800000);
+    tp.tlast = tp.tbefore = -__int_as_float(0x7f800000);
+    tp.ties = 0u;
     tp.endT = 0.f;
 }
 
@@ -1277,7 +1311,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_fwd(DeviceScene S, FwdOut
     if (rec && pix == 0) {
         uint32_t* h = out.tape.base;
         h[0] = kTapeMagic; h[1] = kTapeVersion; h[2] = uint32_t(S.B); h[3] = uint32_t(S.H); h[4] = uint32_t(S.W);
-        h[5] = out.tape.segs; h[6] = S.n; h[7] = S.frame; h[8] = out.tape.cap;
+        h[5] = out.tape.segs; h[6] = S.n; h[7] = S.frame; h[8] = out.tape.cap; h[9] = tape_pool_chunks(paths);
     }
     uint32_t flags = 0u;
     SegTape tp;
@@ -1288,7 +1322,10 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_fwd(DeviceScene S, FwdOut
     path_init(p);
     count(S, C_PATHS);
     uint32_t used = 0u;  // tape hit entries used by this path
-    if (rec) seg_tape_begin(tp, out.tape, pix, paths, used);
+    if (rec) {
+        path_tape_begin(tp, out.tape, pix, paths);
+        seg_tape_begin(tp, used);
+    }
     ray_intersect<K>(S, primary_ray(S, pix), p.cur, rng, false, true, true, out.vis, L, tp,
                      S.tile_primary ? wstack : nullptr);
     if (rec) {
@@ -1325,7 +1362,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_fwd(DeviceScene S, FwdOut
             p.numBounces = depth + 1u;
             sample_next(S, p, rng);
             if (vmax(p.thr) < 1e-4f) break;
-            if (rec) seg_tape_begin(tp, out.tape, pix, paths, used);
+            if (rec) seg_tape_begin(tp, used);
             ray_intersect<K>(S, p.cur.ray, p.cur, rng, true, false, true, out.vis, L, tp);
             if (rec) {
                 store_seg(prec, p.numBounces, p.cur, tp);
@@ -1334,7 +1371,10 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_fwd(DeviceScene S, FwdOut
             }
         }
     }
-    if (rec) prec[0] = flags;
+    if (rec) {
+        prec[0] = flags;
+        prec[3] = tp.chunk;
+    }
     float* o;
     o = out.pbr + 3 * pix; o[0] = p.L.x; o[1] = p.L.y; o[2] = p.L.z;
     o = out.light + 3 * pix; o[0] = p.Lnb.x; o[1] = p.Lnb.y; o[2] = p.Lnb.z;
@@ -1357,7 +1397,8 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_bwd(DeviceScene S, BwdIO 
     if constexpr (TAPE) {
         const uint32_t* h = io.tape.base;
         if (h[0] != kTapeMagic || h[1] != kTapeVersion || h[2] != uint32_t(S.B) || h[3] != uint32_t(S.H) ||
-            h[4] != uint32_t(S.W) || h[5] != io.tape.segs || h[6] != S.n || h[7] != S.frame || h[8] != io.tape.cap) {
+            h[4] != uint32_t(S.W) || h[5] != io.tape.segs || h[6] != S.n || h[7] != S.frame || h[8] != io.tape.cap ||
+            h[9] != tape_pool_chunks(paths)) {
             atomicOr(io.tape_state, 1);  // the replay launch that follows handles every path
             return;
         }
@@ -1365,7 +1406,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_bwd(DeviceScene S, BwdIO 
     } else {
         if (io.replay_mode == 1 && *io.tape_state == 0 && !(prec[0] & kTapeOverflow)) return;
     }
-    const Flush f{&io, pix, paths, &L};
+    const Flush f{&io, pix, paths, &L, prec ? prec[3] : kNoChunk};
     Rng rng;
     rng.seed(Rng::launch_seed(uint32_t(px.x), uint32_t(px.y), uint32_t(px.z), S.frame));
     Path p;

@@ -41,8 +41,10 @@ def _opts(**kw):
 
 def test_tape_size_formula():
     L = _lib.lib()
-    # header 64 words + per path (16 + 16 * segments) words + 8 bytes per hit slot
-    assert L.ptir_tape_bytes(2, 3, 4, 3, 32) == 4 * (64 + 24 * (16 + 48)) + 8 * 24 * 32
+    # header 64 words + per path (16 + 16 * segments) words + 8 bytes per own hit slot, then
+    # a pool of one 128-entry chunk per 64 paths (rounded up)
+    assert L.ptir_tape_bytes(2, 3, 4, 3, 32) == 4 * (64 + 24 * (16 + 48)) + 8 * 24 * 32 + 8 * 128 * 1
+    assert L.ptir_tape_bytes(1, 100, 100, 2, 64) == 4 * (64 + 10000 * (16 + 32)) + 8 * 10000 * 64 + 8 * 128 * 157
     assert L.ptir_tape_bytes(-1, 3, 4, 3, 32) == 0
     assert L.ptir_tape_bytes(1, 1, 1, 3, 0) == 0
 
