@@ -46,8 +46,8 @@ typedef struct ptir_options {
     int32_t enable_metallic;
     int32_t visualize_lights;     /* must be 0 (light visualisation is not built) */
     int32_t kbuffer;              /* hits collected per traversal chunk: 8, 16 (reference) or 32 */
-    int32_t primary_traversal;    /* 0: per-lane chunked traversal for every ray; 1: primary rays of
-                                     an 8x8 pixel tile share one wave-coherent traversal (same hits) */
+    int32_t primary_traversal;    /* 0: every ray walks the hierarchy on its own; 1: the primary rays of
+                                     an 8x8 pixel tile walk it together (same hits, faster) */
 } ptir_options;
 
 /* Scene inputs of one trace call (OptixTracer::trace argument list,
@@ -145,7 +145,6 @@ typedef struct ptir_counters {
     uint64_t paths, segments, candidates, accepted, accepted_primary, shadow_candidates, env_lookups;
     uint64_t mat_atomics, env_atomics;
     uint64_t traversals, nodes, leaf_tests;  /* hierarchy work (no counterpart in the byte model) */
-    uint64_t tile_fallbacks;                 /* primary segments finished by the per-lane traversal */
 } ptir_counters;
 
 /* OptixTracer::OptixTracer (optixTracer.cpp:274-343).  Allocates no device memory. */

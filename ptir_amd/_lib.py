@@ -67,7 +67,7 @@ class BwdOutputs(C.Structure):
 class Counters(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in ("paths", "segments", "candidates", "accepted", "accepted_primary",
                                           "shadow_candidates", "env_lookups", "mat_atomics", "env_atomics",
-                                          "traversals", "nodes", "leaf_tests", "tile_fallbacks")]
+                                          "traversals", "nodes", "leaf_tests")]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}

@@ -106,7 +106,7 @@ ptir::DeviceScene make_scene(ptir_tracer* t, const ptir_scene* sc, const ptir_la
     S.enable_mis = t->opts.enable_mis;
     S.enable_metallic = t->opts.enable_metallic;
     S.has_snormal = sc->shading_normal != nullptr;
-    S.tile_primary = t->opts.primary_traversal;
+    S.wave_primary = t->opts.primary_traversal;
     S.counters = nullptr;
     S.error_flag = b.error_flag;
     return S;
@@ -142,7 +142,6 @@ ptir_status end_counters(ptir_tracer* t, ptir_counters* c, hipStream_t s) {
     c->traversals = h[ptir::C_TRAVERSALS];
     c->nodes = h[ptir::C_NODES];
     c->leaf_tests = h[ptir::C_LEAF_TESTS];
-    c->tile_fallbacks = h[ptir::C_TILE_FALLBACKS];
     if (flag) return fail(t, PTIR_ERR_BVH_DEPTH, "traversal stack overflow");
     return PTIR_OK;
 }

@@ -115,7 +115,7 @@ struct DeviceScene {
     uint32_t max_bounces, render_opts, frame;
     int32_t B, H, W;
     int32_t enable_normals, enable_hitcounts, enable_mis, enable_metallic, has_snormal;
-    int32_t tile_primary;  // primary segments use the wave-coherent tile traversal
+    int32_t wave_primary;  // primary segments use the wave-coherent traversal (wave_collect)
     int32_t tiles_x, tiles_y, num_blocks;  // launch tiling
     unsigned long long* counters;          // 9 counters or null
     int32_t* error_flag;
@@ -187,7 +187,7 @@ struct BwdIO {
 
 enum CounterId { C_PATHS = 0, C_SEGMENTS, C_CANDIDATES, C_ACCEPTED, C_ACCEPTED_PRIMARY, C_SHADOW_CANDIDATES,
                  C_ENV_LOOKUPS, C_MAT_ATOMICS, C_ENV_ATOMICS, C_TRAVERSALS, C_NODES, C_LEAF_TESTS,
-                 C_TILE_FALLBACKS, C_COUNT };
+                 C_COUNT };
 
 __device__ __forceinline__ void count(const DeviceScene& S, int id, unsigned long long v = 1) {
     if (S.counters) atomicAdd(S.counters + id, v);

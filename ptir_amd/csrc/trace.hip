@@ -628,127 +628,99 @@ struct SegTape {
     }
 };
 
-// Wave-coherent front-to-back collection for the rays of one wave (the 8x8 primary tile).
-// The wave walks the hierarchy once with a wave-uniform node stack (node and record
-// loads are uniform); every lane tests the children's boxes with its own ray and keeps
-// its own K-buffer.  A lane may composite its nearest pending candidate as soon as its
-// t is below the lane's safe distance: the smallest box-entry distance of any node still
-// on the stack (kept per lane as a prefix minimum in LDS), because a candidate's peak
-// lies inside its box.  So hits are emitted in exactly increasing (t, slot) order.  A
-// lane whose K-buffer would overflow stops and reports it; the caller finishes that
-// segment with the per-lane chunked traversal from the last emitted hit.
-template <int K, class Proc, class Alive>
-__device__ __forceinline__ bool tile_collect(const DeviceScene& S, const Ray& r, float tmin, float tmax, bool need,
-                                             const Lane& L, uint32_t* wstack, Proc&& proc, Alive&& alive) {
-    const float kInf = __int_as_float(0x7f800000);
+// Wave-coherent variant of collect() for the rays of one wave (an 8x8 pixel tile of
+// primary rays): the wave walks the hierarchy once, visiting a node when any lane's ray
+// enters it before that lane's K-th nearest candidate, with a wave-uniform stack, so node
+// and leaf records are read with uniform (scalar) loads and every lane runs every box
+// and leaf test in lockstep.  Each lane keeps its own K-buffer and applies exactly the
+// per-lane rules of collect() (a leaf counts for a lane only when the lane's ray enters
+// its box before the lane's cut), so every lane gets the K candidates collect() returns.
+// Every active lane of the wave must call it together; `need` = false opts a lane out.
+template <int K>
+__device__ __forceinline__ int wave_collect(const DeviceScene& S, const Ray& r, float tmin, float tmax, bool need,
+                                            const Lane& L, uint32_t* wstack) {
     float bt[K];
     uint32_t bs[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-        bt[j] = kInf;
+        bt[j] = __int_as_float(0x7f800000);
         bs[j] = kInvalid;
     }
-    int cnt = 0;
-    bool ovf = false;
-    const V3 inv = v3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
-    auto insert = [&](float t, uint32_t s) {
-        if (cnt == K) {  // a candidate would be dropped: hand the segment to the chunked traversal
-            ovf = true;
-            return;
-        }
+    // branch-free sorted insertion (all lanes run it together); ok = false leaves it unchanged
+    auto insert = [&](bool ok, float t, uint32_t s) {
+        auto below = [&](int j) { return ok & ((t < bt[j]) | ((t == bt[j]) & (s < bs[j]))); };
+        bool bj = below(K - 1);
 #pragma unroll
-        for (int j = 0; j < K; ++j) {
-            const bool lt = (t < bt[j]) || (t == bt[j] && s < bs[j]);
-            const float tt = lt ? bt[j] : t;
-            const uint32_t ss = lt ? bs[j] : s;
-            bt[j] = lt ? t : bt[j];
-            bs[j] = lt ? s : bs[j];
-            t = tt;
-            s = ss;
+        for (int j = K - 1; j >= 1; --j) {
+            const bool bp = below(j - 1);
+            bt[j] = bp ? bt[j - 1] : (bj ? t : bt[j]);
+            bs[j] = bp ? bs[j - 1] : (bj ? s : bs[j]);
+            bj = bp;
         }
-        ++cnt;
+        bt[0] = bj ? t : bt[0];
+        bs[0] = bj ? s : bs[0];
     };
-    auto leaf = [&](int slot) {
+    auto leaf = [&](bool hit, int slot) {
+        if (hit) count(S, C_LEAF_TESTS);
         float t;
-        count(S, C_LEAF_TESTS);
-        if (cand_test(S, slot, r, tmin, tmax, t)) insert(t, uint32_t(slot));
+        const bool ok = cand_test(S, slot, r, tmin, tmax, t);
+        insert(hit & ok, t, uint32_t(slot));
     };
-    auto emit = [&](float safe) {
-        while (cnt > 0 && bt[0] < safe && alive()) {
-            proc(bt[0], int(bs[0]));
-#pragma unroll
-            for (int j = 0; j < K - 1; ++j) {
-                bt[j] = bt[j + 1];
-                bs[j] = bs[j + 1];
-            }
-            bt[K - 1] = kInf;
-            bs[K - 1] = kInvalid;
-            --cnt;
-        }
-    };
-    count(S, C_TRAVERSALS);
+    if (need) count(S, C_TRAVERSALS);
+    const V3 inv = v3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
     if (S.n == 0) need = false;
-    if (S.root < 0) {
-        if (need) leaf(~S.root);
-        if (need && !ovf) emit(kInf);
-        return ovf;
-    }
-    int link = S.root;
-    int sp = 0;  // wave-uniform
-    for (;;) {
-        need = need && !ovf && alive();
-        if (__ballot(need) == 0) break;
-        link = __builtin_amdgcn_readfirstlane(link);
-        const float4* nd = S.nodes + 4 * link;
-        const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2], n3 = nd[3];
-        count(S, C_NODES);
-        float tl = kInf, tr = kInf;
-        const bool hl = need && slab(n0, n1, r, inv, tmin, tmax, tl);
-        const bool hr = need && slab(n2, n3, r, inv, tmin, tmax, tr);
-        const int cl = __builtin_amdgcn_readfirstlane(__float_as_int(n0.w));
-        const int cr = __builtin_amdgcn_readfirstlane(__float_as_int(n1.w));
-        const uint64_t bl = __ballot(hl), br = __ballot(hr);
-        if (cl < 0 && bl != 0 && hl) leaf(~cl);
-        if (cr < 0 && br != 0 && hr) leaf(~cr);
-        const bool gl = cl >= 0 && bl != 0, gr = cr >= 0 && br != 0;
-        const float tnl = hl ? tl : kInf, tnr = hr ? tr : kInf;
-        const float pm = sp > 0 ? __uint_as_float(L.at(sp - 1)) : kInf;  // min entry over the stack
-        float safe;
-        if (gl && gr) {
-            // near child first, as seen by the first lane that hits both (or either)
-            uint64_t both = __ballot(hl && hr);
-            if (both == 0) both = bl | br;
-            const int leader = __ffsll((long long)both) - 1;
-            const bool lf = __shfl(tnl <= tnr ? 1 : 0, leader) != 0;
-            if (sp >= kStackDepth) {  // cannot happen after the build-time depth check
-                atomicOr(S.error_flag, 1);
-                break;
+    if (S.n > 0 && S.root < 0) {
+        leaf(need, ~S.root);
+    } else if (__ballot(need) != 0) {
+        int link = S.root;  // wave-uniform
+        int sp = 0;         // wave-uniform
+        for (;;) {
+            link = __builtin_amdgcn_readfirstlane(link);
+            const float4* nd = S.nodes + 4 * link;
+            const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2], n3 = nd[3];
+            if (need) count(S, C_NODES);
+            const float cut = fminf(tmax, bt[K - 1]);
+            float tl = 0.f, tr = 0.f;
+            const bool hl = need && slab(n0, n1, r, inv, tmin, cut, tl);
+            const bool hr = need && slab(n2, n3, r, inv, tmin, cut, tr);
+            const int cl = __builtin_amdgcn_readfirstlane(__float_as_int(n0.w));
+            const int cr = __builtin_amdgcn_readfirstlane(__float_as_int(n1.w));
+            const uint64_t bl = __ballot(hl), br = __ballot(hr);
+            if (cl < 0 && bl != 0) leaf(hl, ~cl);
+            if (cr < 0 && br != 0) leaf(hr, ~cr);
+            const bool gl = cl >= 0 && bl != 0, gr = cr >= 0 && br != 0;
+            if (gl && gr) {
+                // near child first, as seen by the first lane that enters both
+                uint64_t both = __ballot(hl && hr);
+                if (both == 0) both = bl;
+                const int leader = __ffsll((long long)both) - 1;
+                const bool lf = __shfl(tl <= tr ? 1 : 0, leader) != 0;
+                if (sp >= kStackDepth) {  // cannot happen after the build-time depth check
+                    atomicOr(S.error_flag, 1);
+                    break;
+                }
+                wstack[sp] = uint32_t(lf ? cr : cl);  // every lane stores the same uniform value
+                ++sp;
+                link = lf ? cl : cr;
+            } else if (gl) {
+                link = cl;
+            } else if (gr) {
+                link = cr;
+            } else {
+                if (sp == 0) break;
+                --sp;
+                link = int(wstack[sp]);
             }
-            const float far_tn = lf ? tnr : tnl;
-            L.at(sp) = __float_as_uint(fminf(pm, far_tn));
-            wstack[sp] = uint32_t(lf ? cr : cl);  // every active lane stores the same uniform link
-            ++sp;
-            link = lf ? cl : cr;
-            safe = fminf(fminf(pm, far_tn), lf ? tnl : tnr);
-        } else if (gl) {
-            link = cl;
-            safe = fminf(pm, tnl);
-        } else if (gr) {
-            link = cr;
-            safe = fminf(pm, tnr);
-        } else if (sp > 0) {
-            --sp;
-            link = int(wstack[sp]);
-            safe = pm;  // includes the popped node's own entry distance
-        } else {
-            link = -1;
-            safe = kInf;
         }
-        if (need && !ovf) emit(safe);
-        if (link < 0) break;
     }
-    if (need && !ovf) emit(kInf);
-    return ovf;
+    int cnt = 0;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        L.at(j) = __float_as_uint(bt[j]);
+        L.at(K + j) = bs[j];
+        cnt += bs[j] != kInvalid ? 1 : 0;
+    }
+    return cnt;
 }
 
 template <int K>
@@ -812,16 +784,22 @@ __device__ __forceinline__ void ray_intersect(const DeviceScene& S, const Ray ra
         pl.lastHit = fmaxf(pl.lastHit, t);
         if (S.enable_hitcounts) pl.hits += acc ? 1.f : 0.f;
     };
-    bool chunked = true;
     if (wstack) {
         // all lanes of the wave arrive here together (the primary segment)
-        const bool need = pl.lastHit <= t1 && pl.T > minT;
-        const bool ovf = tile_collect<K>(S, ray, pl.lastHit + kEpsT, t1 + kEpsT, need, L, wstack, proc,
-                                         [&]() { return pl.T > minT; });
-        chunked = ovf;
-        if (ovf) count(S, C_TILE_FALLBACKS);
-    }
-    if (chunked) {
+        bool done = false;
+        for (;;) {
+            const bool need = !done && pl.lastHit <= t1 && pl.T > minT;
+            if (__ballot(need) == 0) break;
+            const int cnt = wave_collect<K>(S, ray, pl.lastHit + kEpsT, t1 + kEpsT, need, L, wstack);
+            if (need) {
+                if (cnt == 0) done = true;
+                for (int k = 0; k < cnt; ++k) {
+                    if (!(pl.T > minT)) break;
+                    proc(__uint_as_float(L.at(k)), int(L.at(K + k)));
+                }
+            }
+        }
+    } else {
         while (pl.lastHit <= t1 && pl.T > minT) {
             const int cnt = collect<K>(S, ray, pl.lastHit + kEpsT, t1 + kEpsT, L);
             if (cnt == 0) break;
@@ -1297,7 +1275,13 @@ __device__ __forceinline__ void path_init(Path& p) {
 
 // referenceOptix.cu:27-71
 template <int K>
-__global__ void __launch_bounds__(kTraceBlock) k_trace_fwd(DeviceScene S, FwdOut out) {
+// nodes and grec come again as restrict-qualified arguments: the kernel never writes the
+// hierarchy or the records, and saying so lets wave-uniform reads of them become scalar loads
+__global__ void __launch_bounds__(kTraceBlock) k_trace_fwd(DeviceScene S, FwdOut out,
+                                                          const float4* __restrict__ nodes,
+                                                          const float4* __restrict__ grec) {
+    S.nodes = nodes;
+    S.grec = grec;
     __shared__ uint32_t lds[LdsLayout<K>::kWords * kTraceBlock];
     __shared__ uint32_t wst[(kTraceBlock / 64) * kStackDepth];  // wave-uniform node stacks
     const Lane L{lds + threadIdx.x};
@@ -1327,7 +1311,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_fwd(DeviceScene S, FwdOut
         seg_tape_begin(tp, used);
     }
     ray_intersect<K>(S, primary_ray(S, pix), p.cur, rng, false, true, true, out.vis, L, tp,
-                     S.tile_primary ? wstack : nullptr);
+                     S.wave_primary ? wstack : nullptr);
     if (rec) {
         store_seg(prec, 0, p.cur, tp);
         flags |= tp.overflow ? kTapeOverflow : 0u;
@@ -1476,9 +1460,9 @@ hipError_t launch_trace_fwd(const DeviceScene& S0, const FwdOut& out, int kbuf, 
     tiling(S);
     if (S.num_blocks == 0) return hipSuccess;
     switch (kbuf) {
-    case 8: k_trace_fwd<8><<<S.num_blocks, kTraceBlock, 0, s>>>(S, out); break;
-    case 32: k_trace_fwd<32><<<S.num_blocks, kTraceBlock, 0, s>>>(S, out); break;
-    default: k_trace_fwd<16><<<S.num_blocks, kTraceBlock, 0, s>>>(S, out); break;
+    case 8: k_trace_fwd<8><<<S.num_blocks, kTraceBlock, 0, s>>>(S, out, S.nodes, S.grec); break;
+    case 32: k_trace_fwd<32><<<S.num_blocks, kTraceBlock, 0, s>>>(S, out, S.nodes, S.grec); break;
+    default: k_trace_fwd<16><<<S.num_blocks, kTraceBlock, 0, s>>>(S, out, S.nodes, S.grec); break;
     }
     return hipGetLastError();
 }
