@@ -46,8 +46,9 @@ typedef struct ptir_options {
     int32_t enable_metallic;
     int32_t visualize_lights;     /* must be 0 (light visualisation is not built) */
     int32_t kbuffer;              /* hits collected per traversal chunk: 8, 16 (reference) or 32 */
-    int32_t primary_traversal;    /* 0: every ray walks the hierarchy on its own; 1: the primary rays of
-                                     an 8x8 pixel tile walk it together (same hits, faster) */
+    int32_t wave_traversal;       /* 0: every ray walks the hierarchy on its own; 1: the primary rays
+                                     of an 8x8 pixel tile walk it together (the same hits in the
+                                     same order, faster; secondary rays always walk on their own) */
 } ptir_options;
 
 /* Scene inputs of one trace call (OptixTracer::trace argument list,

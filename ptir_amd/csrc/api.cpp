@@ -106,7 +106,7 @@ ptir::DeviceScene make_scene(ptir_tracer* t, const ptir_scene* sc, const ptir_la
     S.enable_mis = t->opts.enable_mis;
     S.enable_metallic = t->opts.enable_metallic;
     S.has_snormal = sc->shading_normal != nullptr;
-    S.wave_primary = t->opts.primary_traversal;
+    S.wave_traversal = t->opts.wave_traversal;
     S.counters = nullptr;
     S.error_flag = b.error_flag;
     return S;
@@ -162,8 +162,8 @@ ptir_status ptir_create(const ptir_options* options, ptir_tracer** out) {
     if (options->visualize_lights) return fail(nullptr, PTIR_ERR_UNSUPPORTED, "visualize_lights is not supported");
     if (options->kbuffer != 8 && options->kbuffer != 16 && options->kbuffer != 32)
         return fail(nullptr, PTIR_ERR_INVALID_ARGUMENT, "kbuffer must be 8, 16 or 32");
-    if (options->primary_traversal != 0 && options->primary_traversal != 1)
-        return fail(nullptr, PTIR_ERR_INVALID_ARGUMENT, "primary_traversal must be 0 or 1");
+    if (options->wave_traversal != 0 && options->wave_traversal != 1)
+        return fail(nullptr, PTIR_ERR_INVALID_ARGUMENT, "wave_traversal must be 0 or 1");
     ptir_tracer* t = new (std::nothrow) ptir_tracer();
     if (!t) return fail(nullptr, PTIR_ERR_OUT_OF_MEMORY, "host allocation failed");
     t->opts = *options;

@@ -115,7 +115,7 @@ struct DeviceScene {
     uint32_t max_bounces, render_opts, frame;
     int32_t B, H, W;
     int32_t enable_normals, enable_hitcounts, enable_mis, enable_metallic, has_snormal;
-    int32_t wave_primary;  // primary segments use the wave-coherent traversal (wave_collect)
+    int32_t wave_traversal;  // primary rays walk the hierarchy per wave (wave_collect)
     int32_t tiles_x, tiles_y, num_blocks;  // launch tiling
     unsigned long long* counters;          // 9 counters or null
     int32_t* error_flag;

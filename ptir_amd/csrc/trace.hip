@@ -1311,7 +1311,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_fwd(DeviceScene S, FwdOut
         seg_tape_begin(tp, used);
     }
     ray_intersect<K>(S, primary_ray(S, pix), p.cur, rng, false, true, true, out.vis, L, tp,
-                     S.wave_primary ? wstack : nullptr);
+                     S.wave_traversal ? wstack : nullptr);
     if (rec) {
         store_seg(prec, 0, p.cur, tp);
         flags |= tp.overflow ? kTapeOverflow : 0u;

@@ -57,17 +57,19 @@ class HipTracer:
                  particle_kernel_density_clamping: bool = True, particle_radiance_sph_degree: int = 3,
                  enable_normals: bool = True, enable_hitcounts: bool = True, enable_mis: bool = True,
                  enable_metallic: bool = False, visualize_lights: bool = False, kbuffer: int = 16,
-                 use_tape: bool = True, tape_hits_per_path: int = 64, primary_traversal: str = "wave"):
+                 use_tape: bool = True, tape_hits_per_path: int = 64, traversal: str = "wave"):
         if pipeline != "reference" or backward_pipeline not in ("referenceBwd", "reference" + "Bwd"):
             raise ValueError(f"unsupported pipeline pair ({pipeline}, {backward_pipeline}); "
                              "only the PTIR 'reference' path tracer is built")
         if primitive != "instances":
             raise ValueError(f"unsupported primitive type {primitive!r}; only 'instances' is built")
+        if traversal not in ("lane", "wave"):
+            raise ValueError(f"traversal must be 'lane' or 'wave', not {traversal!r}")
         o = _lib.Options(float(particle_kernel_degree), float(particle_kernel_min_response),
                          int(bool(particle_kernel_density_clamping)), int(particle_radiance_sph_degree),
                          int(bool(enable_normals)), int(bool(enable_hitcounts)), int(bool(enable_mis)),
                          int(bool(enable_metallic)), int(bool(visualize_lights)), int(kbuffer),
-                         {"lane": 0, "wave": 1}[primary_traversal])
+                         {"lane": 0, "wave": 1}[traversal])
         h = C.c_void_p()
         L = _lib.lib()
         _lib.check(L.ptir_create(C.byref(o), C.byref(h)), None)

@@ -97,9 +97,10 @@ def test_backward_parity_ptir_mode(small, small_run, device):
 def test_forward_is_deterministic_and_kbuffer_invariant(small, device):
     sc, tab = small
     ref = None
-    # the wave-coherent primary traversal and the per-lane one collect the same hits
+    # the wave-coherent primary traversal and the per-lane one collect the same hits in the
+    # same order
     for k, mode in ((8, "wave"), (16, "wave"), (32, "wave"), (16, "lane"), (8, "lane")):
-        tr = HipTracer(enable_metallic=True, kbuffer=k, primary_traversal=mode)
+        tr = HipTracer(enable_metallic=True, kbuffer=k, traversal=mode)
         for _ in range(2):
             outs, _ = gpu_forward(tr, sc, tab, device)
             cur = [o.clone() for o in outs]
