@@ -14,6 +14,7 @@ struct ptir_tracer {
     ptir_options opts;
     ptir::BvhBuffers bvh;
     bool built = false;
+    bool short_stack = false;  // the built hierarchy fits the short traversal stack
     std::string err;
 };
 
@@ -199,8 +200,9 @@ ptir_status ptir_build_bvh(ptir_tracer* t, const float* pos, const float* rot, c
         int32_t depth = 0;
         e = ptir::bvh_depth(t->bvh, s, &depth);
         if (e != hipSuccess) return hip_fail(t, e, "bvh_depth");
-        if (depth + 1 > ptir::kStackDepth)
+        if (depth + 1 > ptir::kStackDeep)
             return fail(t, PTIR_ERR_BVH_DEPTH, "hierarchy depth " + std::to_string(depth) + " exceeds the traversal stack");
+        t->short_stack = depth + 1 <= ptir::kStackShort;
     }
     t->built = true;
     return PTIR_OK;
@@ -235,7 +237,7 @@ ptir_status ptir_trace(ptir_tracer* t, const ptir_scene* sc, const ptir_launch* 
         e = ptir::fill_zero(o.tape.base + ptir::kTapePoolNext, sizeof(uint32_t), s);
         if (e != hipSuccess) return hip_fail(t, e, "zeroing the tape pool counter");
     }
-    e = ptir::launch_trace_fwd(S, o, t->opts.kbuffer, s);
+    e = ptir::launch_trace_fwd(S, o, t->opts.kbuffer, t->short_stack, s);
     if (e != hipSuccess) return hip_fail(t, e, "trace kernel launch");
     return end_counters(t, counters, s);
 }
@@ -280,7 +282,7 @@ ptir_status ptir_trace_bwd(ptir_tracer* t, const ptir_scene* sc, const ptir_laun
         e = ptir::fill_zero(t->bvh.tape_state, sizeof(int32_t), s);
         if (e != hipSuccess) return hip_fail(t, e, "zeroing the tape state");
     }
-    e = ptir::launch_trace_bwd(S, io, t->opts.kbuffer, s);
+    e = ptir::launch_trace_bwd(S, io, t->opts.kbuffer, t->short_stack, s);
     if (e != hipSuccess) return hip_fail(t, e, "trace_bwd kernel launch");
     if (N) {
         e = ptir::scatter_material_grad(t->bvh, out->material_grad, s);

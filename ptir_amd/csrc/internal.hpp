@@ -51,11 +51,12 @@ hipError_t scatter_material_grad(const BvhBuffers& b, float* dmat, hipStream_t s
 // Zero a device buffer with a kernel (safe inside captured graphs, unlike hipMemsetAsync).
 hipError_t fill_zero(void* p, size_t bytes, hipStream_t s);
 
-hipError_t launch_trace_fwd(const DeviceScene& S, const FwdOut& out, int kbuf, hipStream_t s);
-hipError_t launch_trace_bwd(const DeviceScene& S, const BwdIO& io, int kbuf, hipStream_t s);
+hipError_t launch_trace_fwd(const DeviceScene& S, const FwdOut& out, int kbuf, bool short_stack, hipStream_t s);
+hipError_t launch_trace_bwd(const DeviceScene& S, const BwdIO& io, int kbuf, bool short_stack, hipStream_t s);
 
 constexpr int kTraceBlock = 128;  // 2 waves: a 16x8 pixel tile
-constexpr int kStackDepth = 64;   // traversal stack entries per lane (LDS)
+constexpr int kStackDeep = 64;    // traversal stack entries per lane (LDS): any accepted hierarchy
+constexpr int kStackShort = 32;   // hierarchies of depth <= 32 run with this one (more waves fit)
 
 // bytes of a tape with `segs` segments per path and `cap` hits per path
 inline uint64_t tape_bytes(uint64_t paths, uint32_t segs, uint32_t cap) {

@@ -20,9 +20,13 @@ struct Ray {
 };
 
 // ---------------------------------------------------------------- per-lane LDS view
-template <int K>
+// Waves per SIMD the trace kernels are compiled for: four when the per-lane LDS area is
+// the short stack (the LDS then holds four waves per SIMD), else what registers allow.
+constexpr int waves_per_eu(int K, int SD) { return (SD <= kStackShort && 2 * K <= kStackShort) ? 4 : 1; }
+
+template <int K, int SD>
 struct LdsLayout {
-    static constexpr int kWords = (2 * K > kStackDepth) ? 2 * K : kStackDepth;
+    static constexpr int kWords = (2 * K > SD) ? 2 * K : SD;  // per-lane stack of SD entries
 };
 
 struct Lane {
@@ -60,7 +64,7 @@ __device__ __forceinline__ bool slab(float4 lo, float4 hi, const Ray& r, V3 inv,
 
 // Collect the K closest candidates in (tmin, tmax).  Returns the count; the sorted
 // entries are left in LDS: t at words [0,K), leaf slot at words [K,2K).
-template <int K>
+template <int K, int SD>
 __device__ __forceinline__ int collect(const DeviceScene& S, const Ray& r, float tmin, float tmax, const Lane& L) {
     float bt[K];
     uint32_t bs[K];
@@ -124,7 +128,7 @@ __device__ __forceinline__ int collect(const DeviceScene& S, const Ray& r, float
                     if (hr && cr < 0) { push(~cr); hr = false; }
                     if (hl && hr) {
                         const bool lf = tl <= tr;
-                        if (sp >= kStackDepth) {  // cannot happen after the build-time depth check
+                        if (sp >= SD) {  // cannot happen after the build-time depth check
                             atomicOr(S.error_flag, 1);
                             walk = false;
                         } else {
@@ -636,7 +640,7 @@ struct SegTape {
 // per-lane rules of collect() (a leaf counts for a lane only when the lane's ray enters
 // its box before the lane's cut), so every lane gets the K candidates collect() returns.
 // Every active lane of the wave must call it together; `need` = false opts a lane out.
-template <int K>
+template <int K, int SD>
 __device__ __forceinline__ int wave_collect(const DeviceScene& S, const Ray& r, float tmin, float tmax, bool need,
                                             const Lane& L, uint32_t* wstack) {
     float bt[K];
@@ -695,7 +699,7 @@ __device__ __forceinline__ int wave_collect(const DeviceScene& S, const Ray& r, 
                 if (both == 0) both = bl;
                 const int leader = __ffsll((long long)both) - 1;
                 const bool lf = __shfl(tl <= tr ? 1 : 0, leader) != 0;
-                if (sp >= kStackDepth) {  // cannot happen after the build-time depth check
+                if (sp >= kStackDeep) {  // cannot happen after the build-time depth check
                     atomicOr(S.error_flag, 1);
                     break;
                 }
@@ -723,7 +727,7 @@ __device__ __forceinline__ int wave_collect(const DeviceScene& S, const Ray& r, 
     return cnt;
 }
 
-template <int K>
+template <int K, int SD>
 __device__ __forceinline__ void ray_intersect(const DeviceScene& S, const Ray ray, Seg& pl, Rng& rng, bool selfOcc, bool primary,
                               bool gbuffer, uint32_t* vis, const Lane& L, SegTape& tp, uint32_t* wstack = nullptr) {
     float t0, t1;
@@ -790,7 +794,7 @@ __device__ __forceinline__ void ray_intersect(const DeviceScene& S, const Ray ra
         for (;;) {
             const bool need = !done && pl.lastHit <= t1 && pl.T > minT;
             if (__ballot(need) == 0) break;
-            const int cnt = wave_collect<K>(S, ray, pl.lastHit + kEpsT, t1 + kEpsT, need, L, wstack);
+            const int cnt = wave_collect<K, SD>(S, ray, pl.lastHit + kEpsT, t1 + kEpsT, need, L, wstack);
             if (need) {
                 if (cnt == 0) done = true;
                 for (int k = 0; k < cnt; ++k) {
@@ -801,7 +805,7 @@ __device__ __forceinline__ void ray_intersect(const DeviceScene& S, const Ray ra
         }
     } else {
         while (pl.lastHit <= t1 && pl.T > minT) {
-            const int cnt = collect<K>(S, ray, pl.lastHit + kEpsT, t1 + kEpsT, L);
+            const int cnt = collect<K, SD>(S, ray, pl.lastHit + kEpsT, t1 + kEpsT, L);
             if (cnt == 0) break;
             for (int k = 0; k < cnt; ++k) {
                 if (!(pl.T > minT)) break;
@@ -836,7 +840,7 @@ __device__ __forceinline__ void ray_intersect(const DeviceScene& S, const Ray ra
 }
 
 // pathTracer.cuh:161-214 stochastic shadow ray
-template <int K>
+template <int K, int SD>
 __device__ __forceinline__ bool shadow_occluded(const DeviceScene& S, const Ray ray, Rng& rng, float tmin, float tmax, const Lane& L) {
     float t0, t1;
     scene_interval(S, ray, t0, t1);
@@ -844,7 +848,7 @@ __device__ __forceinline__ bool shadow_occluded(const DeviceScene& S, const Ray 
     const float endT = fminf(tmax, t1) + kEpsT;
     if (startT >= endT) return false;
     while (startT < endT) {
-        const int cnt = collect<K>(S, ray, startT + kEpsT, endT, L);
+        const int cnt = collect<K, SD>(S, ray, startT + kEpsT, endT, L);
         if (cnt == 0) break;
         float batchEnd = startT;
         for (int k = 0; k < cnt; ++k) {
@@ -865,7 +869,7 @@ __device__ __forceinline__ bool shadow_occluded(const DeviceScene& S, const Ray 
 }
 
 // pathTracer.cuh:313-390 material-gradient flush over a replayed segment
-template <int K>
+template <int K, int SD>
 __device__ __forceinline__ void intersect_bwd(const DeviceScene& S, const Ray ray, float opacity, float maxHit, V3 ga, float gr,
                               float gm, bool selfOcc, float* dmat_leaf, const Lane& L) {
     const float inv = 1.f / fmaxf(opacity, 1e-12f);
@@ -879,7 +883,7 @@ __device__ __forceinline__ void intersect_bwd(const DeviceScene& S, const Ray ra
     float T = 1.f;
     const float minT = S.min_transmittance;
     while (startT < endT && T > minT) {
-        const int cnt = collect<K>(S, ray, startT + kEpsT, endT, L);
+        const int cnt = collect<K, SD>(S, ray, startT + kEpsT, endT, L);
         if (cnt == 0) break;
         for (int k = 0; k < cnt; ++k) {
             if (!(T > minT)) break;
@@ -908,7 +912,7 @@ __device__ __forceinline__ void intersect_bwd(const DeviceScene& S, const Ray ra
 }
 
 // ---------------------------------------------------------------- path logic
-template <int K>
+template <int K, int SD>
 __device__ __forceinline__ void sample_nee(const DeviceScene& S, Path& p, Rng& rng, const Lane& L) {
     emit_clear(p);
     if (!has_env_light(S)) return;
@@ -923,7 +927,7 @@ __device__ __forceinline__ void sample_nee(const DeviceScene& S, Path& p, Rng& r
     float sPdf;
     const BG bc = eval_ls<false>(wo, n, alb, met, clamp_rough(p.cur.rough), wi, sPdf, S.enable_metallic);
     const Ray lr{p.cur.pos + wi * kSelfOccOffset, wi};
-    if (shadow_occluded<K>(S, lr, rng, 0.f, kInfDist, L)) return;
+    if (shadow_occluded<K, SD>(S, lr, rng, 0.f, kInfDist, L)) return;
     p.eray = lr;
     p.elightPdf = envPdf;
     p.escatterPdf = sPdf;
@@ -1083,23 +1087,23 @@ __device__ __forceinline__ void tape_flush(const DeviceScene& S, const Flush& f,
     }
 }
 
-template <int K, bool TAPE>
+template <int K, int SD, bool TAPE>
 __device__ __forceinline__ void flush_seg(const DeviceScene& S, const Flush& f, const Ray& ray, float opacity, float maxHit,
                                           uint32_t seg, uint32_t cnt, V3 ga, float gr, float gm, bool selfOcc) {
     if constexpr (TAPE) {
         tape_flush(S, f, seg, cnt, opacity, ga, gr, gm);
     } else {
-        intersect_bwd<K>(S, ray, opacity, maxHit, ga, gr, gm, selfOcc, f.io->dmat_leaf, *f.L);
+        intersect_bwd<K, SD>(S, ray, opacity, maxHit, ga, gr, gm, selfOcc, f.io->dmat_leaf, *f.L);
     }
 }
 
-template <int K, bool TAPE>
+template <int K, int SD, bool TAPE>
 __device__ __forceinline__ void pending_bwd(const DeviceScene& S, Path& p, const Flush& f) {
     if (!p.pvalid) return;
     const float dR = dot(p.cur.dirGrad, p.pdir);
     p.pvalid = false;
     if (dR == 0.f) return;
-    flush_seg<K, TAPE>(S, f, p.pray, p.popacity, p.pmaxHit, p.pseg, p.pcnt, v3s(0.f), dR, 0.f, p.pbounces > 1u);
+    flush_seg<K, SD, TAPE>(S, f, p.pray, p.popacity, p.pmaxHit, p.pseg, p.pcnt, v3s(0.f), dR, 0.f, p.pbounces > 1u);
 }
 
 __device__ __forceinline__ void nee_grad_bwd(const DeviceScene& S, Path& p, const Seg& cs) {
@@ -1117,7 +1121,7 @@ __device__ __forceinline__ void nee_grad_bwd(const DeviceScene& S, Path& p, cons
     p.cur.galb += dLdB * bc.dAlb;
 }
 
-template <int K, bool TAPE>
+template <int K, int SD, bool TAPE>
 __device__ __forceinline__ void sample_next_bwd(const DeviceScene& S, Path& p, Rng& rng, const Flush& f) {
     V3 n, wo, alb;
     float met;
@@ -1158,7 +1162,7 @@ __device__ __forceinline__ void sample_next_bwd(const DeviceScene& S, Path& p, R
     p.pbounces = p.numBounces;
     p.pseg = p.cseg;
     p.pcnt = p.ccnt;
-    flush_seg<K, TAPE>(S, f, p.cur.ray, 1.f - p.cur.T, p.cur.lastHit, p.cseg, p.ccnt, p.cur.galb, p.cur.grough,
+    flush_seg<K, SD, TAPE>(S, f, p.cur.ray, 1.f - p.cur.T, p.cur.lastHit, p.cseg, p.ccnt, p.cur.galb, p.cur.grough,
                        p.cur.gmetal, p.numBounces > 1u);
     p.thr = nt;
     const Ray nr{p.cur.pos + safe_normalize(Ld) * kSelfOccOffset, Ld};
@@ -1274,18 +1278,21 @@ __device__ __forceinline__ void path_init(Path& p) {
 }
 
 // referenceOptix.cu:27-71
-template <int K>
+template <int K, int SD>
 // nodes and grec come again as restrict-qualified arguments: the kernel never writes the
 // hierarchy or the records, and saying so lets wave-uniform reads of them become scalar loads
-__global__ void __launch_bounds__(kTraceBlock) k_trace_fwd(DeviceScene S, FwdOut out,
+// With the short stack the LDS allows four waves per SIMD; asking the compiler for four
+// (at most 128 VGPRs, a few spilled) is faster than two waves without spills.
+__global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(waves_per_eu(K, SD))))
+k_trace_fwd(DeviceScene S, FwdOut out,
                                                           const float4* __restrict__ nodes,
                                                           const float4* __restrict__ grec) {
     S.nodes = nodes;
     S.grec = grec;
-    __shared__ uint32_t lds[LdsLayout<K>::kWords * kTraceBlock];
-    __shared__ uint32_t wst[(kTraceBlock / 64) * kStackDepth];  // wave-uniform node stacks
+    __shared__ uint32_t lds[LdsLayout<K, SD>::kWords * kTraceBlock];
+    __shared__ uint32_t wst[(kTraceBlock / 64) * kStackDeep];  // wave-uniform node stacks
     const Lane L{lds + threadIdx.x};
-    uint32_t* wstack = wst + (threadIdx.x / 64) * kStackDepth;
+    uint32_t* wstack = wst + (threadIdx.x / 64) * kStackDeep;
     const PixelId px = pixel_of(S);
     if (!px.ok) return;
     const size_t pix = (size_t(px.z) * S.H + px.y) * S.W + px.x;
@@ -1310,7 +1317,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_fwd(DeviceScene S, FwdOut
         path_tape_begin(tp, out.tape, pix, paths);
         seg_tape_begin(tp, used);
     }
-    ray_intersect<K>(S, primary_ray(S, pix), p.cur, rng, false, true, true, out.vis, L, tp,
+    ray_intersect<K, SD>(S, primary_ray(S, pix), p.cur, rng, false, true, true, out.vis, L, tp,
                      S.wave_traversal ? wstack : nullptr);
     if (rec) {
         store_seg(prec, 0, p.cur, tp);
@@ -1331,7 +1338,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_fwd(DeviceScene S, FwdOut
         out.hits[pix] = c.hits;
     }
     if (p.cur.ivalid) {
-        if (S.enable_mis) sample_nee<K>(S, p, rng, L);
+        if (S.enable_mis) sample_nee<K, SD>(S, p, rng, L);
         if (rec) {
             prec[1] = uint32_t(rng.state);
             prec[2] = uint32_t(rng.state >> 32);
@@ -1347,7 +1354,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_fwd(DeviceScene S, FwdOut
             sample_next(S, p, rng);
             if (vmax(p.thr) < 1e-4f) break;
             if (rec) seg_tape_begin(tp, used);
-            ray_intersect<K>(S, p.cur.ray, p.cur, rng, true, false, true, out.vis, L, tp);
+            ray_intersect<K, SD>(S, p.cur.ray, p.cur, rng, true, false, true, out.vis, L, tp);
             if (rec) {
                 store_seg(prec, p.numBounces, p.cur, tp);
                 flags |= tp.overflow ? kTapeOverflow : 0u;
@@ -1369,9 +1376,10 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_fwd(DeviceScene S, FwdOut
 
 // referenceBwdOptix.cu:27-74.  TAPE = false replays the path by re-tracing every
 // segment; TAPE = true reads the segments and their hits from the forward's tape.
-template <int K, bool TAPE>
-__global__ void __launch_bounds__(kTraceBlock) k_trace_bwd(DeviceScene S, BwdIO io) {
-    __shared__ uint32_t lds[(TAPE ? 1 : LdsLayout<K>::kWords) * kTraceBlock];
+template <int K, int SD, bool TAPE>
+__global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(waves_per_eu(K, TAPE ? 0 : SD))))
+k_trace_bwd(DeviceScene S, BwdIO io) {
+    __shared__ uint32_t lds[(TAPE ? 1 : LdsLayout<K, SD>::kWords) * kTraceBlock];
     const Lane L{lds + threadIdx.x};
     const PixelId px = pixel_of(S);
     if (!px.ok) return;
@@ -1406,7 +1414,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_bwd(DeviceScene S, BwdIO 
     } else {
         SegTape none;
         none.hits = nullptr;
-        ray_intersect<K>(S, primary_ray(S, pix), p.cur, rng, false, true, false, nullptr, L, none);
+        ray_intersect<K, SD>(S, primary_ray(S, pix), p.cur, rng, false, true, false, nullptr, L, none);
     }
     if (!p.cur.ivalid) return;
     if (io.mat_grad) {
@@ -1427,15 +1435,15 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_bwd(DeviceScene S, BwdIO 
             p.econtrib = v3(e1.y, e1.z, e1.w);
         }
     } else {
-        if (S.enable_mis) sample_nee<K>(S, p, rng, L);
+        if (S.enable_mis) sample_nee<K, SD>(S, p, rng, L);
     }
     for (unsigned depth = 0; depth < S.max_bounces && p.active; ++depth) {
         accumulate_bwd(S, p, io.denv);
-        pending_bwd<K, TAPE>(S, p, f);
+        pending_bwd<K, SD, TAPE>(S, p, f);
         p.active &= (depth + 1u < S.max_bounces) && p.cur.ivalid;
         if (!p.active) break;
         p.numBounces = depth + 1u;
-        sample_next_bwd<K, TAPE>(S, p, rng, f);
+        sample_next_bwd<K, SD, TAPE>(S, p, rng, f);
         if (vmax(p.thr) < 1e-4f) break;
         if constexpr (TAPE) {
             p.ccnt = load_seg(prec, p.numBounces, p.cur.ray, p.cur, p.cseg);
@@ -1444,7 +1452,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_bwd(DeviceScene S, BwdIO 
         } else {
             SegTape none;
             none.hits = nullptr;
-            ray_intersect<K>(S, p.cur.ray, p.cur, rng, true, false, false, nullptr, L, none);
+            ray_intersect<K, SD>(S, p.cur.ray, p.cur, rng, true, false, false, nullptr, L, none);
         }
     }
 }
@@ -1455,42 +1463,58 @@ static void tiling(DeviceScene& S) {
     S.num_blocks = S.tiles_x * S.tiles_y * S.B;
 }
 
-hipError_t launch_trace_fwd(const DeviceScene& S0, const FwdOut& out, int kbuf, hipStream_t s) {
-    DeviceScene S = S0;
-    tiling(S);
-    if (S.num_blocks == 0) return hipSuccess;
+// Kernel instances: K = 8 and 16 with the short or the deep stack, K = 32 (whose LDS
+// area is 64 words anyway) with the deep one.
+template <class F>
+static hipError_t dispatch(int kbuf, bool short_stack, F&& f) {
     switch (kbuf) {
-    case 8: k_trace_fwd<8><<<S.num_blocks, kTraceBlock, 0, s>>>(S, out, S.nodes, S.grec); break;
-    case 32: k_trace_fwd<32><<<S.num_blocks, kTraceBlock, 0, s>>>(S, out, S.nodes, S.grec); break;
-    default: k_trace_fwd<16><<<S.num_blocks, kTraceBlock, 0, s>>>(S, out, S.nodes, S.grec); break;
+    case 8: short_stack ? f.template run<8, kStackShort>() : f.template run<8, kStackDeep>(); break;
+    case 32: f.template run<32, kStackDeep>(); break;
+    default: short_stack ? f.template run<16, kStackShort>() : f.template run<16, kStackDeep>(); break;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_trace_bwd(const DeviceScene& S0, const BwdIO& io, int kbuf, hipStream_t s) {
+struct FwdLaunch {
+    const DeviceScene& S;
+    const FwdOut& out;
+    hipStream_t s;
+    template <int K, int SD>
+    void run() const {
+        k_trace_fwd<K, SD><<<S.num_blocks, kTraceBlock, 0, s>>>(S, out, S.nodes, S.grec);
+    }
+};
+
+struct BwdLaunch {
+    const DeviceScene& S;
+    const BwdIO& io;
+    hipStream_t s;
+    template <int K, int SD>
+    void run() const {
+        k_trace_bwd<K, SD, false><<<S.num_blocks, kTraceBlock, 0, s>>>(S, io);
+    }
+};
+
+hipError_t launch_trace_fwd(const DeviceScene& S0, const FwdOut& out, int kbuf, bool short_stack, hipStream_t s) {
     DeviceScene S = S0;
     tiling(S);
     if (S.num_blocks == 0) return hipSuccess;
+    return dispatch(kbuf, short_stack, FwdLaunch{S, out, s});
+}
+
+hipError_t launch_trace_bwd(const DeviceScene& S0, const BwdIO& io, int kbuf, bool short_stack, hipStream_t s) {
+    DeviceScene S = S0;
+    tiling(S);
+    if (S.num_blocks == 0) return hipSuccess;
+    BwdIO r = io;
     if (io.tape.base) {
         // tape pass for every path it covers, then a re-trace pass for the rest
-        k_trace_bwd<16, true><<<S.num_blocks, kTraceBlock, 0, s>>>(S, io);
-        BwdIO r = io;
+        k_trace_bwd<16, kStackDeep, true><<<S.num_blocks, kTraceBlock, 0, s>>>(S, io);
         r.replay_mode = 1;
-        switch (kbuf) {
-        case 8: k_trace_bwd<8, false><<<S.num_blocks, kTraceBlock, 0, s>>>(S, r); break;
-        case 32: k_trace_bwd<32, false><<<S.num_blocks, kTraceBlock, 0, s>>>(S, r); break;
-        default: k_trace_bwd<16, false><<<S.num_blocks, kTraceBlock, 0, s>>>(S, r); break;
-        }
-        return hipGetLastError();
+    } else {
+        r.replay_mode = 0;
     }
-    BwdIO r = io;
-    r.replay_mode = 0;
-    switch (kbuf) {
-    case 8: k_trace_bwd<8, false><<<S.num_blocks, kTraceBlock, 0, s>>>(S, r); break;
-    case 32: k_trace_bwd<32, false><<<S.num_blocks, kTraceBlock, 0, s>>>(S, r); break;
-    default: k_trace_bwd<16, false><<<S.num_blocks, kTraceBlock, 0, s>>>(S, r); break;
-    }
-    return hipGetLastError();
+    return dispatch(kbuf, short_stack, BwdLaunch{S, r, s});
 }
 
 }  // namespace ptir
