@@ -89,7 +89,8 @@ __device__ __forceinline__ int collect(const DeviceScene& S, const Ray& r, float
     auto leaf = [&](int slot) {
         float t;
         count(S, C_LEAF_TESTS);
-        if (cand_test(S, slot, r, tmin, tmax, t)) insert(t, uint32_t(slot));
+        if (cand_test(S, slot, r, tmin, tmax, t) && (t < bt[K - 1] || (t == bt[K - 1] && uint32_t(slot) < bs[K - 1])))
+            insert(t, uint32_t(slot));
     };
     count(S, C_TRAVERSALS);
     const V3 inv = v3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
@@ -668,7 +669,11 @@ __device__ __forceinline__ int wave_collect(const DeviceScene& S, const Ray& r, 
         if (hit) count(S, C_LEAF_TESTS);
         float t;
         const bool ok = cand_test(S, slot, r, tmin, tmax, t);
-        insert(hit & ok, t, uint32_t(slot));
+        // only a candidate that beats the lane's K-th entry changes its buffer; skip the
+        // insertion when no lane has one
+        const uint32_t s = uint32_t(slot);
+        const bool ins = hit & ok & ((t < bt[K - 1]) | ((t == bt[K - 1]) & (s < bs[K - 1])));
+        if (__ballot(ins) != 0) insert(ins, t, s);
     };
     if (need) count(S, C_TRAVERSALS);
     const V3 inv = v3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
