@@ -7,6 +7,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "internal.hpp"
 
@@ -117,7 +118,7 @@ ptir_status begin_counters(ptir_tracer* t, ptir::DeviceScene& S, ptir_counters* 
     hipError_t e = ptir::fill_zero(t->bvh.error_flag, sizeof(int32_t), s);
     if (e != hipSuccess) return hip_fail(t, e, "zeroing the error flag");
     if (!c) return PTIR_OK;
-    e = ptir::fill_zero(t->bvh.counters, ptir::C_COUNT * sizeof(unsigned long long), s);
+    e = ptir::fill_zero(t->bvh.counters, size_t(ptir::kCounterCopies) * ptir::kCounterStride * sizeof(unsigned long long), s);
     if (e != hipSuccess) return hip_fail(t, e, "zeroing counters");
     S.counters = t->bvh.counters;
     return PTIR_OK;
@@ -125,12 +126,16 @@ ptir_status begin_counters(ptir_tracer* t, ptir::DeviceScene& S, ptir_counters* 
 
 ptir_status end_counters(ptir_tracer* t, ptir_counters* c, hipStream_t s) {
     if (!c) return PTIR_OK;
-    unsigned long long h[ptir::C_COUNT];
+    std::vector<unsigned long long> all(size_t(ptir::kCounterCopies) * ptir::kCounterStride);
     int32_t flag = 0;
-    hipError_t e = hipMemcpyAsync(h, t->bvh.counters, sizeof(h), hipMemcpyDeviceToHost, s);
+    hipError_t e = hipMemcpyAsync(all.data(), t->bvh.counters, all.size() * sizeof(unsigned long long),
+                                  hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipMemcpyAsync(&flag, t->bvh.error_flag, sizeof(flag), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_fail(t, e, "reading counters");
+    unsigned long long h[ptir::C_COUNT] = {};
+    for (int k = 0; k < ptir::kCounterCopies; ++k)
+        for (int i = 0; i < ptir::C_COUNT; ++i) h[i] += all[size_t(k) * ptir::kCounterStride + i];
     c->paths = h[ptir::C_PATHS];
     c->segments = h[ptir::C_SEGMENTS];
     c->candidates = h[ptir::C_CANDIDATES];

@@ -294,7 +294,9 @@ hipError_t bvh_reserve(BvhBuffers& b, uint32_t n) {
         if ((e = hipMalloc((void**)&b.bounds_i, 6 * sizeof(int32_t))) != hipSuccess) return e;
         if ((e = hipMalloc((void**)&b.bounds, 6 * sizeof(float))) != hipSuccess) return e;
         if ((e = hipMalloc((void**)&b.depth, sizeof(int32_t))) != hipSuccess) return e;
-        if ((e = hipMalloc((void**)&b.counters, C_COUNT * sizeof(unsigned long long))) != hipSuccess) return e;
+        if ((e = hipMalloc((void**)&b.counters, size_t(kCounterCopies) * kCounterStride * sizeof(unsigned long long))) !=
+            hipSuccess)
+            return e;
         if ((e = hipMalloc((void**)&b.error_flag, sizeof(int32_t))) != hipSuccess) return e;
         if ((e = hipMalloc((void**)&b.tape_state, sizeof(int32_t))) != hipSuccess) return e;
     }

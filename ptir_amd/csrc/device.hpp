@@ -117,7 +117,7 @@ struct DeviceScene {
     int32_t enable_normals, enable_hitcounts, enable_mis, enable_metallic, has_snormal;
     int32_t wave_traversal;  // primary rays walk the hierarchy per wave (wave_collect)
     int32_t tiles_x, tiles_y, num_blocks;  // launch tiling
-    unsigned long long* counters;          // 9 counters or null
+    unsigned long long* counters;          // work counter copies (see count()) or null
     int32_t* error_flag;
 };
 
@@ -189,8 +189,20 @@ enum CounterId { C_PATHS = 0, C_SEGMENTS, C_CANDIDATES, C_ACCEPTED, C_ACCEPTED_P
                  C_ENV_LOOKUPS, C_MAT_ATOMICS, C_ENV_ATOMICS, C_TRAVERSALS, C_NODES, C_LEAF_TESTS,
                  C_COUNT };
 
-__device__ __forceinline__ void count(const DeviceScene& S, int id, unsigned long long v = 1) {
-    if (S.counters) atomicAdd(S.counters + id, v);
+// Work counters live in kCounterCopies copies of kCounterStride entries (one 128-byte
+// line each); a block adds into copy blockIdx % kCounterCopies, and the host sums the
+// copies.  One work unit per calling lane, summed over the wave: one atomic per wave, and
+// spread over many lines, so a counting launch runs about as fast as a plain one.
+constexpr int kCounterStride = 16;
+constexpr int kCounterCopies = 512;
+static_assert(C_COUNT <= kCounterStride, "counter copy too small");
+
+__device__ __forceinline__ void count(const DeviceScene& S, int id) {
+    if (S.counters) {
+        const uint64_t m = __ballot(1);
+        if (__lane_id() == __ffsll((long long)m) - 1)
+            atomicAdd(S.counters + (blockIdx.x % kCounterCopies) * kCounterStride + id, (unsigned long long)__popcll(m));
+    }
 }
 
 // generalized Gaussian response exp(-4.5 (r/3)^(2n)) of squared distance d2 = r^2

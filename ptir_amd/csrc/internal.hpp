@@ -35,7 +35,7 @@ struct BvhBuffers {
     float4* srec = nullptr;          // [2N]
     float4* prec = nullptr;          // [N]
     float* dmat_leaf = nullptr;      // [5N]
-    unsigned long long* counters = nullptr;  // [C_COUNT]
+    unsigned long long* counters = nullptr;  // [kCounterCopies * kCounterStride]
     int32_t* error_flag = nullptr;   // [1]
     int32_t* tape_state = nullptr;   // [1]
 };
