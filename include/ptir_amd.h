@@ -84,6 +84,13 @@ typedef struct ptir_launch {
     int32_t sph_degree;               /* evaluated SH degree (n_active_features) */
     float min_transmittance;
     uint32_t max_bounces;             /* path vertices incl. the primary hit */
+    /* Neural radiance cache (no counterpart in the reference, which has none): 0 = none;
+     * 1 = end every path at its first secondary vertex and write a cache query there (the
+     * cache's answer is added by ptir_cache_query); 2 = trace the whole path and write the
+     * query with its training target (the radiance the rest of the path carried, per unit
+     * throughput).  1 and 2 need max_bounces >= 3 and ptir_fwd_outputs.cache_records.
+     * ptir_trace_bwd must be given the same value. */
+    uint32_t cache_mode;
 } ptir_launch;
 
 /* Forward outputs, all [B,H,W,C] float32 unless noted (optixTracer.cpp:931-944).
@@ -111,6 +118,10 @@ typedef struct ptir_fwd_outputs {
     void* tape;
     uint64_t tape_bytes;
     uint32_t tape_hits_per_path;     /* accepted-hit slots per path, shared by its segments */
+    /* [B,H,W,20] radiance-cache records (cache_mode 1 or 2, else NULL).  Per path:
+     * flag (1 = the path reached its first secondary vertex), position[3], outgoing
+     * direction[3], shading normal[3], roughness, albedo[3], throughput[3], target[3]. */
+    float* cache_records;
 } ptir_fwd_outputs;
 
 /* Backward inputs: forward rayPbr/rayLight and the incoming gradients.  The reference's
@@ -178,6 +189,56 @@ ptir_status ptir_trace_bwd(ptir_tracer* tracer, const ptir_scene* scene, const p
  * reference: it lets ptir_trace_bwd skip re-tracing (a 64 B record per path and per
  * bounce, plus 8 B per hit slot). */
 uint64_t ptir_tape_bytes(int32_t B, int32_t H, int32_t W, uint32_t max_bounces, uint32_t hits_per_path);
+
+/* ------------------------------------------------------------------------------------
+ * Neural radiance cache.  No counterpart in the reference (its tiny-cuda-nn submodule is
+ * empty and unused).  A multiresolution hash grid (16 levels x 2 features, 2^19 entries
+ * per level, resolutions 16..2048 over the scene box) and the SH (bands 0..3) of the
+ * outgoing direction, roughness, albedo and normal feed an MLP of four hidden ReLU layers
+ * of width 64 and a 3-channel exp output, in bf16 on MFMA with FP32 accumulation.  It is
+ * trained online on the records of a cache_mode 2 trace and queried on those of a
+ * cache_mode 1 trace.  The handle owns FP32 master weights, the table, Adam state and
+ * the scratch of one training batch of up to max_batch records. */
+typedef struct ptir_cache ptir_cache;
+
+typedef struct ptir_cache_options {
+    uint64_t max_batch;      /* records per training call */
+    uint32_t seed;           /* initial weights and table */
+    float learning_rate;     /* Adam */
+    float beta1, beta2, epsilon;
+} ptir_cache_options;
+
+ptir_status ptir_cache_create(const ptir_cache_options* options, ptir_cache** out, void* stream);
+ptir_status ptir_cache_destroy(ptir_cache* cache);
+/* Sizes of the FP32 parameters: weights W0..W3 [64,64] (out, in) then W4 [16,64] (rows 3..15
+ * zero); table [16, 2^19, 2]. */
+uint64_t ptir_cache_num_weights(void);
+uint64_t ptir_cache_table_floats(void);
+/* Device-to-device copies of the parameters out of / into the cache (either pointer may be
+ * NULL).  Writing repacks the bf16 operands. */
+ptir_status ptir_cache_read_params(ptir_cache* cache, float* weights, float* table, void* stream);
+ptir_status ptir_cache_write_params(ptir_cache* cache, const float* weights, const float* table, void* stream);
+/* Gradients left by the last ptir_cache_backprop (cleared by ptir_cache_step) and its loss
+ * (device [1]); any pointer may be NULL. */
+ptir_status ptir_cache_read_grads(ptir_cache* cache, float* weight_grad, float* table_grad, float* loss,
+                                  void* stream);
+/* Cache answers for n records (scene_box: device [6] lo.xyz hi.xyz).  For every record
+ * with flag 1, adds throughput * answer to pbr [n,3] and to the indirect half of
+ * pbr_components [n,2,3] (either may be NULL); prediction [n,3] (or NULL) receives the
+ * answer of every record with flag 1 (other rows are left as they are). */
+ptir_status ptir_cache_query(ptir_cache* cache, const float* records, uint64_t n, const float* scene_box, float* pbr,
+                             float* pbr_components, float* prediction, void* stream);
+/* Loss and gradients of one training batch of n <= max_batch records (flag 0 records are
+ * ignored; the loss is the mean over all n): overwrites weight_grad and loss, adds into
+ * table_grad. */
+ptir_status ptir_cache_backprop(ptir_cache* cache, const float* records, uint64_t n, const float* scene_box,
+                                void* stream);
+/* One Adam step on weights and table; clears both gradients. */
+ptir_status ptir_cache_step(ptir_cache* cache, void* stream);
+/* ptir_cache_backprop followed by ptir_cache_step. */
+ptir_status ptir_cache_train(ptir_cache* cache, const float* records, uint64_t n, const float* scene_box,
+                             void* stream);
+const char* ptir_cache_last_error(const ptir_cache* cache);
 
 /* Human-readable text of the last error on this handle (or of the last ptir_create
  * failure when tracer is NULL).  Valid until the next call on the handle. */

@@ -23,7 +23,9 @@ PTIR_STATUS = {
 }
 
 EXPORTED = ["ptir_create", "ptir_destroy", "ptir_build_bvh", "ptir_trace", "ptir_trace_bwd", "ptir_tape_bytes",
-            "ptir_last_error", "ptir_version"]
+            "ptir_last_error", "ptir_version", "ptir_cache_create", "ptir_cache_destroy", "ptir_cache_num_weights",
+            "ptir_cache_table_floats", "ptir_cache_read_params", "ptir_cache_write_params", "ptir_cache_read_grads",
+            "ptir_cache_query", "ptir_cache_backprop", "ptir_cache_step", "ptir_cache_train", "ptir_cache_last_error"]
 
 
 class Options(C.Structure):
@@ -44,14 +46,15 @@ class Scene(C.Structure):
 class Launch(C.Structure):
     _fields_ = [("frame", C.c_uint32), ("ray_to_world", C.c_void_p), ("ray_ori", C.c_void_p), ("ray_dir", C.c_void_p),
                 ("B", C.c_int32), ("H", C.c_int32), ("W", C.c_int32), ("render_opts", C.c_uint32),
-                ("sph_degree", C.c_int32), ("min_transmittance", C.c_float), ("max_bounces", C.c_uint32)]
+                ("sph_degree", C.c_int32), ("min_transmittance", C.c_float), ("max_bounces", C.c_uint32),
+                ("cache_mode", C.c_uint32)]
 
 
 class FwdOutputs(C.Structure):
     _fields_ = [(k, C.c_void_p) for k in ("radiance", "density", "hit_distance", "hit_distance2", "distortion",
                                           "normal", "shading_normal", "material", "hits_count", "visibility", "pbr",
                                           "light", "pbr_components", "tape")] + [
-        ("tape_bytes", C.c_uint64), ("tape_hits_per_path", C.c_uint32)]
+        ("tape_bytes", C.c_uint64), ("tape_hits_per_path", C.c_uint32), ("cache_records", C.c_void_p)]
 
 
 class BwdInputs(C.Structure):
@@ -71,6 +74,11 @@ class Counters(C.Structure):
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+class CacheOptions(C.Structure):
+    _fields_ = [("max_batch", C.c_uint64), ("seed", C.c_uint32), ("learning_rate", C.c_float), ("beta1", C.c_float),
+                ("beta2", C.c_float), ("epsilon", C.c_float)]
 
 
 _lib = None
@@ -97,7 +105,23 @@ def lib() -> C.CDLL:
         L.ptir_last_error.argtypes = [C.c_void_p]
         L.ptir_last_error.restype = C.c_char_p
         L.ptir_version.restype = C.c_char_p
-        for name in ("ptir_create", "ptir_destroy", "ptir_build_bvh", "ptir_trace", "ptir_trace_bwd"):
+        vp, u64 = C.c_void_p, C.c_uint64
+        L.ptir_cache_create.argtypes = [C.POINTER(CacheOptions), C.POINTER(C.c_void_p), vp]
+        L.ptir_cache_destroy.argtypes = [vp]
+        L.ptir_cache_num_weights.restype = u64
+        L.ptir_cache_table_floats.restype = u64
+        L.ptir_cache_read_params.argtypes = [vp, vp, vp, vp]
+        L.ptir_cache_write_params.argtypes = [vp, vp, vp, vp]
+        L.ptir_cache_read_grads.argtypes = [vp, vp, vp, vp, vp]
+        L.ptir_cache_query.argtypes = [vp, vp, u64, vp, vp, vp, vp, vp]
+        L.ptir_cache_backprop.argtypes = [vp, vp, u64, vp, vp]
+        L.ptir_cache_step.argtypes = [vp, vp]
+        L.ptir_cache_train.argtypes = [vp, vp, u64, vp, vp]
+        L.ptir_cache_last_error.argtypes = [vp]
+        L.ptir_cache_last_error.restype = C.c_char_p
+        for name in ("ptir_create", "ptir_destroy", "ptir_build_bvh", "ptir_trace", "ptir_trace_bwd",
+                     "ptir_cache_create", "ptir_cache_destroy", "ptir_cache_read_params", "ptir_cache_write_params",
+                     "ptir_cache_read_grads", "ptir_cache_query", "ptir_cache_backprop", "ptir_cache_step", "ptir_cache_train"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -112,4 +136,10 @@ class PtirError(RuntimeError):
 def check(status: int, handle) -> None:
     if status != 0:
         msg = lib().ptir_last_error(handle)
+        raise PtirError(status, msg.decode() if msg else "")
+
+
+def check_cache(status: int, handle) -> None:
+    if status != 0:
+        msg = lib().ptir_cache_last_error(handle)
         raise PtirError(status, msg.decode() if msg else "")

@@ -58,6 +58,9 @@ ptir_status check_scene(ptir_tracer* t, const ptir_scene* sc, const ptir_launch*
         return fail(t, PTIR_ERR_INVALID_ARGUMENT, "null ray tensor");
     if (paths > (1ll << 31) - 1) return fail(t, PTIR_ERR_INVALID_ARGUMENT, "more than 2^31-1 paths in one launch");
     if (la->render_opts > 1u) return fail(t, PTIR_ERR_INVALID_ARGUMENT, "render_opts must be 0 or 1");
+    if (la->cache_mode > 2u) return fail(t, PTIR_ERR_INVALID_ARGUMENT, "cache_mode must be 0, 1 or 2");
+    if (la->cache_mode && (la->max_bounces < 3u || la->render_opts != 0u))
+        return fail(t, PTIR_ERR_INVALID_ARGUMENT, "cache_mode 1 and 2 need max_bounces >= 3 and render_opts 0");
     if (la->sph_degree < 0 || la->sph_degree > 3) return fail(t, PTIR_ERR_INVALID_ARGUMENT, "sph_degree must be in [0,3]");
     const int ncoef = (la->sph_degree + 1) * (la->sph_degree + 1);
     if (sc->sh_stride < 3 * ncoef)
@@ -99,6 +102,7 @@ ptir::DeviceScene make_scene(ptir_tracer* t, const ptir_scene* sc, const ptir_la
     // OptixTracer::trace: maxBounces = renderOpts ? min(maxBounces, 2) : maxBounces (optixTracer.cpp:959)
     S.max_bounces = la->render_opts ? (la->max_bounces < 2u ? la->max_bounces : 2u) : la->max_bounces;
     S.render_opts = la->render_opts;
+    S.cache_mode = la->cache_mode;
     S.frame = la->frame;
     S.B = la->B;
     S.H = la->H;
@@ -230,7 +234,10 @@ ptir_status ptir_trace(ptir_tracer* t, const ptir_scene* sc, const ptir_launch* 
     if ((st = begin_counters(t, S, counters, s)) != PTIR_OK) return st;
     ptir::FwdOut o{out->radiance, out->density, out->hit_distance, out->hit_distance2, out->distortion, out->normal,
                    out->shading_normal, out->material, out->hits_count, out->pbr, out->light, out->pbr_components,
-                   reinterpret_cast<uint32_t*>(out->visibility), ptir::Tape{nullptr, 0u, 0u}};
+                   reinterpret_cast<uint32_t*>(out->visibility), ptir::Tape{nullptr, 0u, 0u}, nullptr};
+    if (la->cache_mode && !out->cache_records)
+        return fail(t, PTIR_ERR_INVALID_ARGUMENT, "cache_mode needs cache_records");
+    if (la->cache_mode) o.cache = out->cache_records;
     if (out->tape) {
         const uint64_t need = ptir::tape_bytes((uint64_t)la->B * la->H * la->W, S.max_bounces, out->tape_hits_per_path);
         if (out->tape_hits_per_path == 0 || out->tape_hits_per_path > 0xFFFFu - ptir::kTapeChunk ||

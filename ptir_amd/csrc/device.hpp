@@ -113,6 +113,7 @@ struct DeviceScene {
     int32_t kernel_degree;
     float min_response, min_transmittance;
     uint32_t max_bounces, render_opts, frame;
+    uint32_t cache_mode;     // 0 none, 1 end paths at the first secondary vertex with a cache query, 2 record targets
     int32_t B, H, W;
     int32_t enable_normals, enable_hitcounts, enable_mis, enable_metallic, has_snormal;
     int32_t wave_traversal;  // primary rays walk the hierarchy per wave (wave_collect)
@@ -174,6 +175,7 @@ struct FwdOut {
     float *rad, *dns, *hit, *hit2, *dist, *nrm, *snrm, *mat, *hits, *pbr, *light, *comp;
     uint32_t* vis;  // [N] original order
     Tape tape;
+    float* cache;   // [B,H,W,20] radiance-cache records (null: none)
 };
 
 struct BwdIO {

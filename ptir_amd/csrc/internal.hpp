@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cmath>
+
 #include "device.hpp"
 
 namespace ptir {
@@ -63,5 +65,54 @@ inline uint64_t tape_bytes(uint64_t paths, uint32_t segs, uint32_t cap) {
     const uint64_t rec = uint64_t(kTapePathWords) + uint64_t(kTapeSegWords) * segs;
     return 4ull * (kTapeHeaderWords + paths * rec) + 8ull * paths * cap + 8ull * kTapeChunk * tape_pool_chunks(paths);
 }
+
+// ---------------------------------------------------------------- neural radiance cache
+namespace nrc {
+constexpr int kLevels = 16;            // hash-grid levels, 2 features each
+constexpr int kLog2Table = 19;
+constexpr int kTableSize = 1 << kLog2Table;  // entries per level
+constexpr int kHidden = 4;             // hidden layers of width 64
+constexpr int kOut = 3;
+constexpr int kOutOffset = kHidden * 4096;   // output layer [16 rows (3 used), 64] after the hidden ones
+constexpr int kParams = kOutOffset + 16 * 64;
+constexpr int kFrags = 70;             // packed bf16 operand fragments (forward 34, backward 36)
+constexpr float kZMax = 15.f;          // exp output: exponent clamp
+constexpr float kLossEps = 0.01f;      // relative L2: (p - t)^2 / (p^2 + eps)
+// query record, float32 x kRecFloats, written by the tracer at the first secondary vertex
+constexpr int kRecFloats = 20;
+constexpr int kRecFlag = 0, kRecPos = 1, kRecDir = 4, kRecNrm = 7, kRecRough = 10, kRecAlb = 11, kRecThr = 14,
+              kRecTarget = 17;
+
+struct AdamParams {
+    float lr, beta1, beta2, eps;
+};
+}  // namespace nrc
+
+// resolution of hash-grid level l: round(16 * 128^(l/15)), 16 ... 2048
+inline int nrc_level_res(int l) {
+    return int(floor(16.0 * exp(double(l) * log(2048.0 / 16.0) / double(nrc::kLevels - 1)) + 0.5));
+}
+inline uint64_t nrc_table_floats() { return uint64_t(nrc::kLevels) * nrc::kTableSize * 2; }
+
+struct NrcBuffers {
+    uint64_t max_batch = 0;
+    float *w = nullptr, *dw = nullptr, *mw = nullptr, *vw = nullptr;               // [kParams] FP32 master + Adam
+    float *table = nullptr, *dtable = nullptr, *mt = nullptr, *vt = nullptr;       // [kLevels, kTableSize, 2]
+    void* frags = nullptr;          // [kFrags, 64] x 16 B packed bf16 operands
+    void* act = nullptr;            // [5, max_batch, 64] bf16 activations of a training batch
+    float* z = nullptr;             // [max_batch, 4] output pre-activations
+    float* dx = nullptr;            // [max_batch, 32] hash-feature gradients
+    float* dw_part = nullptr;       // [tiles, kParams] per-tile weight gradients
+    float* loss = nullptr;          // [1]
+    uint32_t* step = nullptr;       // [1] Adam step counter
+};
+
+hipError_t nrc_alloc(NrcBuffers& b, uint64_t max_batch, uint32_t seed, hipStream_t s);
+void nrc_free(NrcBuffers& b);
+hipError_t nrc_pack(NrcBuffers& b, hipStream_t s);
+hipError_t nrc_query(const NrcBuffers& b, const float* rec, uint64_t n, const float* box, float* pbr, float* comp,
+                     float* pred, hipStream_t s);
+hipError_t nrc_backprop(NrcBuffers& b, const float* rec, uint64_t n, const float* box, hipStream_t s);
+hipError_t nrc_step(NrcBuffers& b, const nrc::AdamParams& h, hipStream_t s);
 
 }  // namespace ptir

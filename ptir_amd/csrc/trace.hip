@@ -1230,6 +1230,25 @@ __device__ __forceinline__ void seg_tape_begin(SegTape& tp, uint32_t used) {
     tp.endT = 0.f;
 }
 
+// ---------------------------------------------------------------- radiance-cache records
+// At the first secondary vertex: the query (position, outgoing direction, shading normal,
+// roughness, albedo as the BRDF sees them) and the path throughput that scales the cache's
+// answer.  For training, the target slot holds L so far; the end of the path turns it into
+// the suffix radiance per unit throughput.
+__device__ __forceinline__ void cache_record(const DeviceScene& S, const Path& p, float* r) {
+    V3 n, wo, alb;
+    float met;
+    shading_frame(S, p.cur, n, wo, alb, met);
+    r[nrc::kRecFlag] = 1.f;
+    r[nrc::kRecPos] = p.cur.pos.x; r[nrc::kRecPos + 1] = p.cur.pos.y; r[nrc::kRecPos + 2] = p.cur.pos.z;
+    r[nrc::kRecDir] = wo.x; r[nrc::kRecDir + 1] = wo.y; r[nrc::kRecDir + 2] = wo.z;
+    r[nrc::kRecNrm] = n.x; r[nrc::kRecNrm + 1] = n.y; r[nrc::kRecNrm + 2] = n.z;
+    r[nrc::kRecRough] = clamp_rough(p.cur.rough);
+    r[nrc::kRecAlb] = alb.x; r[nrc::kRecAlb + 1] = alb.y; r[nrc::kRecAlb + 2] = alb.z;
+    r[nrc::kRecThr] = p.thr.x; r[nrc::kRecThr + 1] = p.thr.y; r[nrc::kRecThr + 2] = p.thr.z;
+    r[nrc::kRecTarget] = p.L.x; r[nrc::kRecTarget + 1] = p.L.y; r[nrc::kRecTarget + 2] = p.L.z;
+}
+
 // ---------------------------------------------------------------- launch mapping
 struct PixelId {
     int x, y, z;
@@ -1312,6 +1331,8 @@ k_trace_fwd(DeviceScene S, FwdOut out,
     uint32_t flags = 0u;
     SegTape tp;
     tp.hits = nullptr;
+    float* crec = out.cache ? out.cache + size_t(nrc::kRecFloats) * pix : nullptr;
+    if (crec) crec[nrc::kRecFlag] = 0.f;
     Rng rng;
     rng.seed(Rng::launch_seed(uint32_t(px.x), uint32_t(px.y), uint32_t(px.z), S.frame));
     Path p;
@@ -1356,6 +1377,10 @@ k_trace_fwd(DeviceScene S, FwdOut out,
             p.active &= (depth + 1u < S.max_bounces) && p.cur.ivalid;
             if (!p.active) break;
             p.numBounces = depth + 1u;
+            if (crec && p.numBounces == 2u) {
+                cache_record(S, p, crec);
+                if (S.cache_mode == 1u) break;  // the cache stands in for the rest of the path
+            }
             sample_next(S, p, rng);
             if (vmax(p.thr) < 1e-4f) break;
             if (rec) seg_tape_begin(tp, used);
@@ -1370,6 +1395,14 @@ k_trace_fwd(DeviceScene S, FwdOut out,
     if (rec) {
         prec[0] = flags;
         prec[3] = tp.chunk;
+    }
+    if (crec && S.cache_mode == 2u && crec[nrc::kRecFlag] != 0.f) {
+        // training target: the radiance the rest of the path carried, per unit throughput at the vertex
+        float* tg = crec + nrc::kRecTarget;
+        const float* th = crec + nrc::kRecThr;
+        tg[0] = th[0] > 0.f ? (p.L.x - tg[0]) / th[0] : 0.f;
+        tg[1] = th[1] > 0.f ? (p.L.y - tg[1]) / th[1] : 0.f;
+        tg[2] = th[2] > 0.f ? (p.L.z - tg[2]) / th[2] : 0.f;
     }
     float* o;
     o = out.pbr + 3 * pix; o[0] = p.L.x; o[1] = p.L.y; o[2] = p.L.z;
@@ -1448,6 +1481,7 @@ k_trace_bwd(DeviceScene S, BwdIO io) {
         p.active &= (depth + 1u < S.max_bounces) && p.cur.ivalid;
         if (!p.active) break;
         p.numBounces = depth + 1u;
+        if (S.cache_mode == 1u && p.numBounces == 2u) break;  // the forward ended here with a cache query
         sample_next_bwd<K, SD, TAPE>(S, p, rng, f);
         if (vmax(p.thr) < 1e-4f) break;
         if constexpr (TAPE) {

@@ -153,7 +153,8 @@ class HipTracer:
         return sc, keep, env
 
     @staticmethod
-    def _launch(frame, ray_to_world, ray_ori, ray_dir, render_opts, sph_degree, min_transmittance, max_bounces):
+    def _launch(frame, ray_to_world, ray_ori, ray_dir, render_opts, sph_degree, min_transmittance, max_bounces,
+                cache_mode=0):
         ro = _need(ray_ori, "ray_ori", (3,))
         rd = _need(ray_dir, "ray_dir", (3,))
         if ro.dim() != 4 or ro.shape != rd.shape:
@@ -169,6 +170,7 @@ class HipTracer:
         la.B, la.H, la.W = ro.shape[0], ro.shape[1], ro.shape[2]
         la.render_opts, la.sph_degree = int(render_opts), int(sph_degree)
         la.min_transmittance, la.max_bounces = float(min_transmittance), int(max_bounces)
+        la.cache_mode = int(cache_mode)
         return la, [ro, rd, T]
 
     # -------------------------------------------------------------- trace
@@ -176,14 +178,19 @@ class HipTracer:
               particle_radiance, particle_shading_normal, environment, environment_alias_table, lights=None,
               light_alias_table=None, mesh_light_vertices=None, mesh_light_triangles=None, mesh_lights=None,
               mesh_light_triangle_alias_table=None, render_opts=0, sph_degree=3, min_transmittance=1e-3,
-              max_bounces=3, counters=False, tape: torch.Tensor | None = None):
+              max_bounces=3, counters=False, tape: torch.Tensor | None = None, cache_mode: int = 0,
+              cache_records: torch.Tensor | None = None):
         """OptixTracer::trace (optixTracer.cpp:909-1013): returns the 13 outputs
         (rad, dns, hit[2], hit2, distortion, nrm, snrm, material, hits, visibility, pbr,
-        light, pbr_components)."""
+        light, pbr_components).
+
+        cache_mode / cache_records (no reference counterpart): 1 ends every path at its first
+        secondary vertex and writes a radiance-cache query there, 2 traces on and writes the
+        query with its training target; cache_records is [B,H,W,20] float32."""
         sc, keep, _ = self._scene(particle_density, particle_material, particle_radiance, particle_shading_normal,
                                   environment, environment_alias_table, lights, light_alias_table, mesh_lights)
         la, keep2 = self._launch(frame_id, ray_to_world, ray_ori, ray_dir, render_opts, sph_degree,
-                                 min_transmittance, max_bounces)
+                                 min_transmittance, max_bounces, cache_mode)
         B, H, W = la.B, la.H, la.W
         dev = ray_dir.device
         f = dict(dtype=torch.float32, device=dev)
@@ -204,6 +211,12 @@ class HipTracer:
         if tape is not None:
             tape = _need(tape, "tape", dtype=torch.uint8)
             out.tape, out.tape_bytes, out.tape_hits_per_path = _ptr(tape), tape.numel(), self.tape_hits_per_path
+        if cache_records is not None:
+            if tuple(cache_records.shape) != (B, H, W, 20):
+                raise ValueError(f"cache_records must be [{B},{H},{W},20], got {tuple(cache_records.shape)}")
+            if not cache_records.is_contiguous():
+                raise ValueError("cache_records must be contiguous (the tracer writes into it)")
+            out.cache_records = _ptr(_need(cache_records, "cache_records"))
         cnt = _lib.Counters() if counters else None
         st = _lib.lib().ptir_trace(self._h, C.byref(sc), C.byref(la), C.byref(out),
                                    C.byref(cnt) if cnt is not None else None, _stream())
@@ -220,13 +233,16 @@ class HipTracer:
                   light_alias_table, mesh_light_vertices, mesh_light_triangles, mesh_lights,
                   mesh_light_triangle_alias_table, ray_rad_grd, ray_dns_grd, ray_hit_grd, ray_hit2_grd, ray_dist_grd,
                   ray_nrm_grd, ray_snrm_grd, ray_mat_grd, ray_pbr_grd, ray_light_grd, render_opts=0, sph_degree=3,
-                  min_transmittance=1e-3, max_bounces=3, counters=False, tape: torch.Tensor | None = None):
+                  min_transmittance=1e-3, max_bounces=3, counters=False, tape: torch.Tensor | None = None,
+                  cache_mode: int = 0):
         """OptixTracer::traceBwd (optixTracer.cpp:1015-1145): returns (dDensity [N,12],
-        dMaterial [N,5], dSH, dShadingNormal [N,3], dEnv [H,W,4])."""
+        dMaterial [N,5], dSH, dShadingNormal [N,3], dEnv [H,W,4]).  cache_mode must be the
+        forward's (with 1 the replay ends at the first secondary vertex, where the forward
+        handed the path to the radiance cache)."""
         sc, keep, env = self._scene(particle_density, particle_material, particle_radiance, particle_shading_normal,
                                     environment, environment_alias_table, lights, light_alias_table, mesh_lights)
         la, keep2 = self._launch(frame_id, ray_to_world, ray_ori, ray_dir, render_opts, sph_degree,
-                                 min_transmittance, max_bounces)
+                                 min_transmittance, max_bounces, cache_mode)
         B, H, W = la.B, la.H, la.W
 
         def pix(t, name, c, allow_none=False):
