@@ -11,20 +11,33 @@
 // Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
 // this library, and only as the checker.  The product (ptir_amd/) never calls it.
 //
-// Pinning status (see tests/test_oracle_*.py):
+// Pinning status (tests/test_oracle_pins.py, test_oracle_closed_forms.py,
+// test_oracle_backward_fd.py):
 //   pcg32, launch seed ........ pinned: published pcg32 KAT + closed form
 //   particle response ......... pinned: closed form exp(-r^4/18), r=3 special case
 //   compositing ............... pinned: single/disjoint Gaussian closed forms,
 //                                brute-force vs BVH candidate sets
-//   SH ........................ pinned: textbook real SH (orthonormality by quadrature)
+//   Gaussian geometry ......... pinned: rotated anisotropic Gaussians vs the quadratic
+//                                form of Sigma^-1 (quaternion convention, peak, rho);
+//                                primary normal vs the 3-sigma ellipsoid entry point
+//                                (unrotated Gaussians; the rotated case is restated as
+//                                written, gaussianParticles.cuh:433-435)
+//   SH ........................ pinned: every coefficient vs scipy's complex SH in the
+//                                3DGS real convention; orthonormality by quadrature
 //   BRDF eval/sample/grads .... pinned: pdf integrates to 1, sample histogram,
 //                                grads vs central differences of the value
 //   env lookup/sampling ....... pinned: constant env, bilinear partition of unity,
 //                                alias-table histogram vs pdf, integral of pdf = 1
-//   full forward .............. pinned: closed-form single-plane scenes (see tests)
-//   ptir backward ............. parity unpinned against the reference (no reference
-//                                vectors exist); pinned only by the structural tests
-//                                (env-grad sum identity, linearity in the loss grad).
+//   full forward .............. pinned: closed-form plane scenes: a Lambert+GGX disc
+//                                under a constant sky (NEE + MIS, one bounce), an
+//                                INDIRECT floor under an emissive ceiling, a two-bounce
+//                                floor + wall under a constant sky (direct and indirect)
+//   ptir backward ............. pinned for albedo (MIS on and off), metallic and env
+//                                texels (MIS off) by central differences of the FP64
+//                                forward; linear in the output gradients.  Its
+//                                roughness gradient (a forward-difference direction term
+//                                by design) and the NEE-side terms the reference leaves
+//                                disabled are restated as written: parity unpinned.
 //
 // Candidate set.  The reference gets candidates from OptiX: instance k is a unit cube
 // under M_k = R_k diag(kscl_k) (src/particlePrimitives.cu:543-610) and the IS program

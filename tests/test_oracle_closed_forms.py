@@ -1,0 +1,360 @@
+"""Closed-form pins of the oracle's forward: SH radiance, rotated anisotropic Gaussians,
+the INDIRECT render option and a two-bounce scene.
+
+Every expected value here is derived from the mathematics, not from the oracle's code:
+SH from scipy's complex spherical harmonics, the Gaussian's peak and Mahalanobis distance
+from a quadratic-form minimisation, and the multi-bounce expectations from hemisphere
+quadrature of a textbook BRDF over analytic planes.
+"""
+
+import math
+
+import numpy as np
+import pytest
+import scipy.special as sps
+import torch
+
+from oracle import oracle as O
+from ptir_amd import envmap
+
+
+# ----------------------------------------------------------------------------- helpers
+def _gaussians(rows):
+    """rows: (pos3, sigma, quat4 wxyz, scale3) -> [N,12] particle density tensor."""
+    d = torch.zeros(len(rows), 12)
+    for i, (p, s, q, sc) in enumerate(rows):
+        d[i, 0:3] = torch.tensor(p, dtype=torch.float32)
+        d[i, 3] = s
+        d[i, 4:8] = torch.tensor(q, dtype=torch.float32)
+        d[i, 8:11] = torch.tensor(sc, dtype=torch.float32)
+    return d
+
+
+def _rays(o, d):
+    o = torch.as_tensor(np.asarray(o, dtype=np.float32)).reshape(1, 1, -1, 3)
+    d = torch.as_tensor(np.asarray(d, dtype=np.float32)).reshape(1, 1, -1, 3)
+    return O.Rays(o, d, torch.eye(4))
+
+
+def _quat_to_matrix(q):
+    """Rotation matrix of a unit quaternion (w, x, y, z), active, column-vector convention."""
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+def _real_sh(l, m, d):
+    """Real SH of the 3DGS convention: sqrt(2) Re / Im of the complex Y_l^|m| (with the
+    Condon-Shortley phase scipy includes) for m > 0 / m < 0, Y_l^0 for m = 0."""
+    d = np.asarray(d, dtype=np.float64)
+    theta = math.acos(max(-1.0, min(1.0, d[2])))
+    phi = math.atan2(d[1], d[0])
+    y = complex(sps.sph_harm_y(l, abs(m), theta, phi))
+    if m > 0:
+        return math.sqrt(2) * y.real
+    if m < 0:
+        return math.sqrt(2) * y.imag
+    return y.real
+
+
+def _textbook_brdf(n, wo, L, albedo, rough):
+    """Lambert + Schlick Fresnel (F0 = 0.04) + GGX D + Smith-Schlick G (k = r^2/2), metallic 0.
+    Returns (f cos, mixed sampling pdf 0.5 (cos/pi + D NH / (4 VH))), vectorised over L."""
+    NL = np.clip(L @ n, 0, 1)
+    NV = float(np.clip(np.dot(wo, n), 0, 1))
+    H = L + wo
+    H /= np.linalg.norm(H, axis=-1, keepdims=True)
+    NH = np.clip(H @ n, 0, 1)
+    VH = np.clip(H @ wo, 0, 1)
+    F = 0.04 + 0.96 * (1 - VH) ** 5
+    a2 = rough ** 4
+    D = a2 / (math.pi * (NH ** 2 * (a2 - 1) + 1) ** 2)
+    k = rough ** 2 / 2
+    G = (NV / (NV * (1 - k) + k)) * (NL / (NL * (1 - k) + k))
+    fcos = albedo * (1 - F) * NL / math.pi + F * D * G / (4 * NV)
+    pdf = 0.5 * (NL / math.pi + D * NH / (4 * np.maximum(VH, 1e-12)))
+    return np.where(NL > 0, fcos, 0.0), np.where(NL > 0, pdf, 0.0)
+
+
+def _hemisphere(n, n_theta, n_phi):
+    """Midpoint grid of equal-solid-angle cells on the hemisphere around unit n."""
+    mu = (np.arange(n_theta) + 0.5) / n_theta
+    ph = (np.arange(n_phi) + 0.5) / n_phi * 2 * np.pi
+    M, P = np.meshgrid(mu, ph, indexing="ij")
+    s = np.sqrt(1 - M ** 2)
+    local = np.stack([s * np.cos(P), s * np.sin(P), M], -1).reshape(-1, 3)
+    t = np.array([1.0, 0, 0]) if abs(n[0]) < 0.9 else np.array([0, 1.0, 0])
+    b1 = np.cross(n, t)
+    b1 /= np.linalg.norm(b1)
+    b2 = np.cross(n, b1)
+    return local @ np.stack([b1, b2, n]), 2 * np.pi / (n_theta * n_phi)
+
+
+def _p_ggx_reflection_up(wo, n, rough, m=2_000_000, seed=11):
+    """P(the GGX-D half-vector reflection of wo lies above the surface), by sampling the
+    textbook inversion cos(theta_h) = sqrt((1-u)/(1+(a^2-1)u)) in the frame of n."""
+    rng = np.random.default_rng(seed)
+    xi, ph = rng.uniform(size=m), rng.uniform(size=m) * 2 * np.pi
+    a2 = rough ** 4
+    ct = np.sqrt((1 - xi) / (1 + (a2 - 1) * xi))
+    st = np.sqrt(1 - ct ** 2)
+    t = np.array([1.0, 0, 0]) if abs(n[0]) < 0.9 else np.array([0, 1.0, 0])
+    b1 = np.cross(n, t)
+    b1 /= np.linalg.norm(b1)
+    b2 = np.cross(n, b1)
+    H = (st * np.cos(ph))[:, None] * b1 + (st * np.sin(ph))[:, None] * b2 + ct[:, None] * n
+    vh = H @ wo
+    L = 2 * vh[:, None] * H - wo
+    return float(np.mean((vh > 0) & (L @ n > 0)))
+
+
+# ----------------------------------------------------------------------------- SH
+@pytest.mark.parametrize("degree", [0, 1, 2, 3])
+def test_sh_radiance_of_every_coefficient_matches_complex_spherical_harmonics(degree):
+    """One Gaussian crossed through its centre (response 1, weight = sigma): with a single
+    non-zero SH coefficient c in the red channel, rad = sigma max(c Y_k(d) + 1/2, 0)
+    (gaussianParticles.cuh:55-87) and the other channels see 1/2.  Coefficients above the
+    evaluated degree are ignored."""
+    sigma = 0.8
+    dens = _gaussians([((0.1, -0.2, 0.3), sigma, (1, 0, 0, 0), (0.05, 0.05, 0.05))])
+    rng = np.random.default_rng(degree)
+    dirs = rng.normal(size=(6, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    mu = np.array([0.1, -0.2, 0.3])
+    k = 0
+    for l in range(4):
+        for m in range(-l, l + 1):
+            sh = torch.zeros(1, 48)
+            c = 1.3 if k % 2 == 0 else -0.9
+            sh[0, 3 * k] = c
+            inp = O.Inputs(dens, torch.full((1, 5), 0.5), sh, torch.tensor([[0.0, 0.0, 1.0]]), None, None,
+                           sph_degree=degree, max_bounces=1)
+            out = O.trace(inp, _rays(mu - 2 * dirs, dirs), nthreads=1)
+            for j, d in enumerate(dirs):
+                # the oracle evaluates the degree-l polynomial at the float32 direction,
+                # which is |d|^l Y(d/|d|) for a homogeneous harmonic polynomial
+                d32 = np.float32(d).astype(np.float64)
+                nd = np.linalg.norm(d32)
+                y = _real_sh(l, m, d32 / nd) * nd ** l if l <= degree else 0.0
+                expect = sigma * max(c * y + 0.5, 0.0)
+                got = out["rad"][0, 0, j]
+                assert got[0] == pytest.approx(expect, rel=1e-6, abs=1e-9), (l, m, d)
+                np.testing.assert_allclose(got[1:], [0.5 * sigma] * 2, rtol=1e-7)
+            k += 1
+
+
+def test_sh_basis_is_orthonormal_by_quadrature():
+    """The 16 basis functions read back from the oracle (unit coefficients) are orthonormal
+    over the sphere: a check of the normalisation constants independent of the sign
+    convention."""
+    # Gauss-Legendre in cos(theta) x uniform in phi: exact for the degree-6 products
+    n_t, n_p = 12, 24
+    mu, wmu = np.polynomial.legendre.leggauss(n_t)
+    ph = (np.arange(n_p) + 0.5) / n_p * 2 * np.pi
+    M, P = np.meshgrid(mu, ph, indexing="ij")
+    s = np.sqrt(1 - M ** 2)
+    dirs = np.stack([s * np.cos(P), s * np.sin(P), M], -1).reshape(-1, 3)
+    dw = (np.repeat(wmu, n_p) * (2 * np.pi / n_p))
+    dens = _gaussians([((0, 0, 0), 0.5, (1, 0, 0, 0), (0.05, 0.05, 0.05))])
+    basis = []
+    for k in range(16):
+        sh = torch.zeros(1, 48)
+        sh[0, 3 * k] = 0.2  # small enough that 0.2 Y + 0.5 > 0 everywhere (|Y| < 2.5)
+        inp = O.Inputs(dens, torch.full((1, 5), 0.5), sh, torch.tensor([[0.0, 0.0, 1.0]]), None, None,
+                       sph_degree=3, max_bounces=1)
+        out = O.trace(inp, _rays(-2 * dirs, dirs), nthreads=4)
+        basis.append((out["rad"][0, 0, :, 0] / 0.5 - 0.5) / 0.2)
+    B = np.stack(basis)
+    G = (B * dw) @ B.T
+    np.testing.assert_allclose(G, np.eye(16), atol=1e-5)
+
+
+# ----------------------------------------------------------------------------- Gaussian geometry
+def test_rotated_anisotropic_gaussian_matches_quadratic_form():
+    """Peak and Mahalanobis distance of a ray through a rotated anisotropic Gaussian from
+    the quadratic form of Sigma^-1 = R diag(1/s^2) R^T: along o + t d,
+    t* = -d^T S (o - mu) / d^T S d and rho^2 = (o-mu)^T S (o-mu) - (d^T S (o-mu))^2 / d^T S d.
+    Then opacity = min(0.99, sigma exp(-rho^4/18)) and depth = opacity t* |d|.  A transposed
+    rotation, a wrong quaternion order or a wrong scale axis changes both."""
+    rng = np.random.default_rng(5)
+    checked = 0
+    for trial in range(40):
+        q = rng.normal(size=4)
+        q /= np.linalg.norm(q)
+        s = np.array([0.12, 0.05, 0.3]) * rng.uniform(0.7, 1.3, 3)
+        sigma = float(rng.uniform(0.3, 0.95))
+        mu = rng.uniform(-0.5, 0.5, 3)
+        R = _quat_to_matrix(q)
+        S = R @ np.diag(1 / s ** 2) @ R.T
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        # aim at a point offset from the centre by a random fraction of the scale ellipsoid
+        off = R @ (s * rng.normal(size=3) * rng.uniform(0.0, 1.2))
+        o = mu + off - 3.0 * d
+        dens = _gaussians([(mu, sigma, q, s)])
+        inp = O.Inputs(dens, torch.full((1, 5), 0.5), torch.zeros(1, 48), torch.tensor([[0.0, 0.0, 1.0]]), None, None,
+                       max_bounces=1)
+        out = O.trace(inp, _rays(o, d), nthreads=1)
+        # the oracle reads float32 inputs: use the rounded values in the closed form
+        o32, d32 = np.float32(o).astype(np.float64), np.float32(d).astype(np.float64)
+        mu32, s32, q32 = (np.float32(v).astype(np.float64) for v in (mu, s, q))
+        R32 = _quat_to_matrix(q32)
+        S = R32 @ np.diag(1 / s32 ** 2) @ R32.T
+        a = o32 - mu32
+        dsd, dsa, asa = d32 @ S @ d32, d32 @ S @ a, a @ S @ a
+        t = -dsa / dsd
+        rho2 = asa - dsa ** 2 / dsd
+        g = math.exp(-rho2 ** 2 / 18)
+        alpha = min(0.99, np.float32(sigma) * g)
+        ks = O.kernel_scale(float(np.float32(sigma)))
+        q_peak = np.diag(1 / s32) @ R32.T @ (a + t * d32)  # peak point in the Gaussian's frame
+        inside = np.max(np.abs(q_peak)) <= ks
+        accepted = inside and g > 0.0113 and alpha > 1 / 255
+        if not accepted or np.max(np.abs(q_peak)) > 0.98 * ks:
+            continue  # keep away from the candidate-box boundary, where rounding decides
+        checked += 1
+        assert out["dns"][0, 0, 0, 0] == pytest.approx(alpha, rel=1e-6), trial
+        assert out["hit"][0, 0, 0, 0] == pytest.approx(alpha * t * np.linalg.norm(d32), rel=1e-6), trial
+        assert out["hit"][0, 0, 0, 1] == pytest.approx(t, rel=1e-6), trial
+        assert out["hit2"][0, 0, 0, 0] == pytest.approx(alpha * t * t, rel=1e-6), trial
+    assert checked >= 15
+
+
+def test_axis_aligned_gaussian_normal_points_to_the_entry_point_of_the_3sigma_ellipsoid():
+    """For an unrotated Gaussian the primary-hit normal (gaussianParticles.cuh:433-435) is
+    the direction from the centre to where the ray enters the ellipsoid
+    (x - mu)^T diag(1/s^2) (x - mu) = 9, weighted by the opacity."""
+    rng = np.random.default_rng(6)
+    for trial in range(20):
+        s = rng.uniform(0.03, 0.3, 3)
+        mu = rng.uniform(-0.3, 0.3, 3)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        o = mu + s * rng.uniform(-1.5, 1.5, 3) - 4.0 * d
+        dens = _gaussians([(mu, 0.9, (1, 0, 0, 0), s)])
+        inp = O.Inputs(dens, torch.full((1, 5), 0.5), torch.zeros(1, 48), torch.tensor([[0.0, 0.0, 1.0]]), None, None,
+                       max_bounces=1)
+        out = O.trace(inp, _rays(o, d), nthreads=1)
+        alpha = out["dns"][0, 0, 0, 0]
+        if alpha == 0:
+            continue
+        o32, d32, mu32, s32 = (np.float32(v).astype(np.float64) for v in (o, d, mu, s))
+        # ray / ellipsoid: |diag(1/s)(o + t d - mu)|^2 = 9, entry = smaller root
+        A = (d32 / s32) @ (d32 / s32)
+        B = 2 * ((o32 - mu32) / s32) @ (d32 / s32)
+        Cc = ((o32 - mu32) / s32) @ ((o32 - mu32) / s32) - 9
+        t_e = (-B - math.sqrt(B * B - 4 * A * Cc)) / (2 * A)
+        nrm = o32 + t_e * d32 - mu32
+        nrm /= np.linalg.norm(nrm)
+        np.testing.assert_allclose(out["nrm"][0, 0, 0], alpha * nrm, rtol=1e-5, atol=1e-7)
+
+
+# ----------------------------------------------------------------------------- multi-bounce closed forms
+BIG = 1000.0  # in-plane scale of the "infinite" plane Gaussians
+
+
+def _plane(center, normal_axis, sigma=0.99):
+    s = [BIG, BIG, BIG]
+    s[normal_axis] = 1e-3
+    return (center, sigma, (1.0, 0.0, 0.0, 0.0), s)
+
+
+def _down_rays(res, z, x0, x1, y0, y1):
+    xs = x0 + (np.arange(res) + 0.5) / res * (x1 - x0)
+    ys = y0 + (np.arange(res) + 0.5) / res * (y1 - y0)
+    X, Y = np.meshgrid(xs, ys, indexing="ij")
+    ori = np.stack([X, Y, np.full_like(X, z)], -1).reshape(1, res, res, 3).astype(np.float32)
+    dirs = np.broadcast_to(np.array([0, 0, -1.0], np.float32), ori.shape).copy()
+    return O.Rays(torch.from_numpy(ori), torch.from_numpy(dirs), torch.eye(4))
+
+
+def test_indirect_option_under_an_emissive_ceiling_matches_quadrature():
+    """render_opts = INDIRECT (environment.cuh:650-675): the secondary vertex's SH radiance
+    is the light.  A Lambert+GGX floor (z = 0) under an opaque ceiling (z = 1) whose SH has
+    only the DC term c0, no environment: every BRDF sample hits the ceiling, whose
+    composited radiance is alpha (Y_00 c0 + 1/2), so
+
+        E[pbr] = alpha (Y_00 c0 + 1/2) * integral over the hemisphere of f cos,
+
+    and all of it is indirect.  (The clamp of maxBounces to 2 under this option is host
+    code, optixTracer.cpp:959, and is checked on the GPU path.)"""
+    c0, albedo, rough = 1.5, 0.6, 1.0
+    dens = _gaussians([_plane((0, 0, 0), 2), _plane((0, 0, 1), 2)])
+    mat = torch.tensor([[albedo] * 3 + [rough, 0.0]] * 2)
+    sh = torch.zeros(2, 48)
+    sh[1, 0:3] = c0
+    sn = torch.tensor([[0.0, 0.0, 1.0], [0.0, 0.0, -1.0]])
+    rays = _down_rays(48, 0.5, -0.2, 0.2, -0.2, 0.2)
+    acc = np.zeros((3, 3))
+    frames = 24
+    for f in range(frames):
+        inp = O.Inputs(dens, mat, sh, sn, None, None, render_opts=1, max_bounces=2, frame=f)
+        out = O.trace(inp, rays, nthreads=8)
+        acc[0] += out["pbr"].reshape(-1, 3).mean(0)
+        acc[1] += out["comp"][..., 0, :].reshape(-1, 3).mean(0)
+        acc[2] += out["comp"][..., 1, :].reshape(-1, 3).mean(0)
+    est = acc / frames
+    n, wo = np.array([0, 0, 1.0]), np.array([0, 0, 1.0])
+    L, dw = _hemisphere(n, 256, 256)
+    fcos, _ = _textbook_brdf(n, wo, L, albedo, rough)
+    radiance = 0.99 * (0.28209479177387814 * c0 + 0.5)
+    expect = radiance * float(fcos.sum() * dw)
+    np.testing.assert_allclose(est[0], [expect] * 3, rtol=0.02)
+    np.testing.assert_allclose(est[1], [0, 0, 0], atol=1e-12)        # direct part
+    np.testing.assert_allclose(est[2], est[0], rtol=1e-12)           # all indirect
+
+
+def test_two_bounce_wall_and_floor_under_constant_sky_matches_quadrature():
+    """maxBounces = 3 (two bounces): an opaque Lambert+GGX floor z = 0 (x > 0 side seen by
+    the camera) meets an opaque wall x = 0 (normal +x), under a constant sky E.  Camera
+    rays hit the floor at x0 > 0 head-on.  From the floor a direction escapes iff its x
+    component is >= 0, else it hits the wall; from the wall a direction escapes iff its z
+    component is > 0, else it hits the floor and the path ends (maxBounces reached).
+    These visibility rules do not depend on x0, so every pixel has the expectation
+
+      direct   = E [ integral_{w.x>=0} f0 cos (w_b + P_ok w_l) dw + P_ok integral_{w.x<0} f0 cos w_l (1-a) dw ]
+      indirect = E   integral_{w.x<0} f0 cos  J(-w) dw,   J(wo) = integral_{v.x>0, v.z>0} f1(wo, v) cos dv
+
+    with the power-heuristic weights of the first bounce (w_b, w_l with p_l = 1/4pi for a
+    constant sky), no MIS at the second bounce, the NEE term kept only when the BRDF sample
+    after it survives (P_ok, environment.cuh:636-648) and a shadow ray toward the wall
+    passing with probability 1 - a (a = 0.99, stochastic occlusion, pathTracer.cuh:161-214)."""
+    E, albedo, rough = 1.3, 0.6, 1.0
+    dens = _gaussians([_plane((0, 0, 0), 2), _plane((0, 0, 0), 0)])
+    mat = torch.tensor([[albedo] * 3 + [rough, 0.0]] * 2)
+    sn = torch.tensor([[0.0, 0.0, 1.0], [1.0, 0.0, 0.0]])
+    env = torch.ones(64, 128, 4) * E
+    tab = envmap.pack_alias_table(envmap.build_environment_alias_table(env))
+    rays = _down_rays(64, 3.0, 0.3, 0.7, -0.2, 0.2)
+    frames = 24
+    acc = np.zeros((3, 3))
+    for f in range(frames):
+        inp = O.Inputs(dens, mat, torch.zeros(2, 48), sn, env, tab, max_bounces=3, frame=f)
+        out = O.trace(inp, rays, nthreads=8)
+        acc[0] += out["pbr"].reshape(-1, 3).mean(0)
+        acc[1] += out["comp"][..., 0, :].reshape(-1, 3).mean(0)
+        acc[2] += out["comp"][..., 1, :].reshape(-1, 3).mean(0)
+    est = acc / frames
+
+    nf, wo = np.array([0, 0, 1.0]), np.array([0, 0, 1.0])
+    W1, dw1 = _hemisphere(nf, 160, 320)
+    f1, pb = _textbook_brdf(nf, wo, W1, albedo, rough)
+    pl = 1 / (4 * math.pi)
+    wl = pl ** 2 / (pl ** 2 + pb ** 2)
+    wb = 1 - wl
+    p_ok = 0.5 + 0.5 * _p_ggx_reflection_up(wo, nf, rough)
+    out_side = W1[:, 0] >= 0
+    direct = E * float(((f1 * (wb + p_ok * wl))[out_side]).sum() * dw1
+                       + (p_ok * (f1 * wl * (1 - 0.99))[~out_side]).sum() * dw1)
+    nw = np.array([1.0, 0, 0])
+    W2, dw2 = _hemisphere(nw, 48, 96)
+    up = W2[:, 2] > 0
+    J = np.zeros(W1.shape[0])
+    for i in np.nonzero(~out_side)[0]:
+        f2, _ = _textbook_brdf(nw, -W1[i], W2[up], albedo, rough)
+        J[i] = f2.sum() * dw2
+    indirect = E * float((f1 * J)[~out_side].sum() * dw1)
+    np.testing.assert_allclose(est[1], [direct] * 3, rtol=0.025)
+    np.testing.assert_allclose(est[2], [indirect] * 3, rtol=0.04)
+    np.testing.assert_allclose(est[0], est[1] + est[2], rtol=1e-12)
