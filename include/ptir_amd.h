@@ -63,12 +63,17 @@ typedef struct ptir_scene {
     const float* shading_normal;      /* [N,3] */
     const float* environment;         /* [env_h, env_w, 4] or NULL */
     int32_t env_h, env_w;
-    const float* env_table;           /* [3, tab_h, tab_w] (prob, alias, pdf) or NULL */
+    const float* env_table;           /* [3, tab_h, tab_w] in the tab_kind layout, or NULL */
     int32_t tab_h, tab_w;
     uint32_t num_lights;              /* lights [L,9]; must be 0 */
     uint32_t num_light_entries;       /* lightAliasTable [5,E]; must be 0 */
     uint32_t num_mesh_lights;         /* meshLights [M,8]; must be 0 */
+    int32_t tab_kind;                 /* layout of env_table: PTIR_ENV_TABLE_ALIAS (the reference's
+                                         (prob, alias, pdf), built by the caller) or
+                                         PTIR_ENV_TABLE_CDF (ptir_env_table_build) */
 } ptir_scene;
+
+enum { PTIR_ENV_TABLE_ALIAS = 0, PTIR_ENV_TABLE_CDF = 1 };
 
 /* Launch description.  rays are [B,H,W,3]; ray_to_world is a DEVICE pointer to a
  * [*,4,4] float32 matrix of which only the first 12 floats are used (the reference reads
@@ -240,8 +245,28 @@ ptir_status ptir_cache_train(ptir_cache* cache, const float* records, uint64_t n
                              void* stream);
 const char* ptir_cache_last_error(const ptir_cache* cache);
 
-/* Human-readable text of the last error on this handle (or of the last ptir_create
- * failure when tracer is NULL).  Valid until the next call on the handle. */
+/* ------------------------------------------------------------------------------------
+ * Environment importance table built on the device (layout PTIR_ENV_TABLE_CDF).
+ * Replaces the caller-side alias table build (threedgrut/model/aliastable.py:244-297,
+ * packed by threedgptir_tracer/tracer.py:819-847): the same cell weights (area resampling
+ * of the [env_h, env_w, 4] environment to tab_h x tab_w, Rec.709 luminance of the clamped
+ * RGB, times the equirect cell solid angle; solid angles alone when the environment carries
+ * no finite positive light) and the same per-steradian pdf w / (sum(w) * solid_angle), but
+ * sampled by CDF inversion instead of a Vose alias table, with the prefix sums in FP64.
+ * table is device [3, tab_h, tab_w] float32: plane 0 the inclusive conditional CDF of each
+ * row, plane 1 column 0 the inclusive marginal CDF of the rows (other columns zero), plane 2
+ * the pdf.  Every CDF ends at exactly 1.  The sampler draws four numbers like the alias
+ * sampler (row, column, position in the cell), so the rest of a path's random stream is the
+ * same under either layout; the chosen cells differ, so parity with the alias sampler is
+ * distributional.  Scratch of ptir_env_table_scratch_bytes() is allocated stream-ordered
+ * (hipMallocAsync / hipFreeAsync) on the stream; nothing synchronises.  Errors are
+ * reported by ptir_last_error(NULL). */
+uint64_t ptir_env_table_scratch_bytes(int32_t tab_h, int32_t tab_w);
+ptir_status ptir_env_table_build(const float* environment, int32_t env_h, int32_t env_w, int32_t tab_h, int32_t tab_w,
+                                 float* table, void* stream);
+
+/* Human-readable text of the last error on this handle (or, when tracer is NULL, of the
+ * last failed call without a handle on this thread: ptir_create, the table builders).  Valid until the next call on the handle. */
 const char* ptir_last_error(const ptir_tracer* tracer);
 
 /* Library version string. */

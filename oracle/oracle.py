@@ -31,6 +31,7 @@ class OracleScene(C.Structure):
         ("sph_degree", C.c_int32), ("enable_normals", C.c_int32), ("enable_hitcounts", C.c_int32),
         ("enable_mis", C.c_int32), ("enable_metallic", C.c_int32), ("render_opts", C.c_int32),
         ("min_transmittance", C.c_float), ("max_bounces", C.c_uint32), ("frame", C.c_uint32), ("kbuf", C.c_int32),
+        ("tab_kind", C.c_int32),
     ]
 
 
@@ -120,7 +121,7 @@ class Inputs:
     def __init__(self, density, material, sh, snormal, env, table, *, sph_degree=3, kernel_degree=4.0,
                  min_response=0.0113, density_clamping=True, enable_normals=True, enable_hitcounts=True,
                  enable_mis=True, enable_metallic=False, render_opts=0, min_transmittance=1e-3, max_bounces=3,
-                 frame=0, kbuf=16):
+                 frame=0, kbuf=16, tab_kind=0):
         self.density = _f32(density).reshape(-1, 12)
         self.material = _f32(material).reshape(-1, 5)
         self.sh = _f32(sh).reshape(self.density.shape[0], -1)
@@ -141,6 +142,7 @@ class Inputs:
         s.enable_normals, s.enable_hitcounts = int(enable_normals), int(enable_hitcounts)
         s.enable_mis, s.enable_metallic, s.render_opts = int(enable_mis), int(enable_metallic), int(render_opts)
         s.min_transmittance, s.max_bounces, s.frame, s.kbuf = min_transmittance, max_bounces, frame, kbuf
+        s.tab_kind = int(tab_kind)
         self.s = s
 
     @classmethod
@@ -273,3 +275,40 @@ def scene_aabb(inp: Inputs):
     out = np.zeros(6)
     lib().oracle_scene_aabb(C.byref(inp.s), out.ctypes.data)
     return out
+
+
+def env_cdf_table(env, h=64, w=128):
+    """CPU restatement, in float64, of the device-built environment CDF table
+    (ptir_env_table_build): weights of threedgrut/model/aliastable.py:244-297 (area
+    resampling, Rec.709 luminance of the clamped RGB, times the equirect cell solid angle),
+    per-steradian pdf w / (sum(w) sa), inclusive conditional CDF per row (plane 0), inclusive
+    marginal CDF of the rows in column 0 of plane 1, pdf in plane 2; float32 [3, h, w]."""
+    e = np.asarray(env.detach().cpu().numpy() if hasattr(env, "detach") else env, dtype=np.float64)[..., :3]
+    H, W = e.shape[:2]
+    rgb = np.zeros((h, w, 3))
+    for y in range(h):
+        y0, y1 = (y * H) // h, -((-(y + 1) * H) // h)
+        for x in range(w):
+            x0, x1 = (x * W) // w, -((-(x + 1) * W) // w)
+            rgb[y, x] = e[y0:y1, x0:x1].reshape(-1, 3).mean(0)
+    rgb = np.where(np.isfinite(rgb) & (rgb > 0), rgb, 0.0)
+    lum = rgb @ np.array([0.2126, 0.7152, 0.0722])
+    edges = (np.arange(h + 1) / h - 0.5) * np.pi
+    sa_row = np.clip((2 * np.pi / w) * (np.sin(edges[1:]) - np.sin(edges[:-1])), 0, None)
+    sa = np.repeat(sa_row[:, None], w, axis=1)
+    wts = lum * sa
+    total = wts.sum()
+    if not (np.isfinite(total) and total > 0):
+        wts = sa.copy()
+        total = wts.sum()
+    rows = wts.sum(1)
+    out = np.zeros((3, h, w))
+    with np.errstate(invalid="ignore", divide="ignore"):
+        cond = np.where(rows[:, None] > 0, np.cumsum(wts, 1) / rows[:, None], (np.arange(w) + 1.0) / w)
+    cond[:, -1] = 1.0
+    marg = np.cumsum(rows) / total
+    marg[-1] = 1.0
+    out[0] = cond
+    out[1, :, 0] = marg
+    out[2] = np.where(sa > 0, wts / (total * np.where(sa > 0, sa, 1.0)), 0.0)
+    return out.astype(np.float32)

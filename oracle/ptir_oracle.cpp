@@ -169,6 +169,7 @@ struct OracleScene {
     uint32_t max_bounces;
     uint32_t frame;
     int32_t kbuf;            // hits per chunk (16 in the reference)
+    int32_t tab_kind;        // 0: alias table (prob, alias, pdf); 1: CDF table (conditional, marginal, pdf)
 };
 
 struct OracleRays {
@@ -751,15 +752,28 @@ struct Tracer {
         }
         return bg;
     }
-    // environment.cuh:227-255 alias sampling (4 draws, left-to-right argument order)
+    // environment.cuh:227-255 alias sampling (4 draws, left-to-right argument order).  With a
+    // CDF table the first two draws invert the row marginal and the row's conditional CDF:
+    // the chosen index is the first whose inclusive CDF exceeds the draw (a linear scan here).
     V3<R> sample_env(Pcg32& rng, R& pdf) {
         const int nc = in.tab_h * in.tab_w;
-        const float* prob = in.tab;
-        const float* alias = in.tab + nc;
         const float* pdfs = in.tab + 2 * nc;
-        const int init = std::min(int(R(rng.next_1d()) * R(nc)), nc - 1);
-        const int al = std::max(0, std::min(int(R(alias[init]) + R(0.5)), nc - 1));
-        const int cell = R(rng.next_1d()) < R(prob[init]) ? init : al;
+        int cell;
+        if (in.tab_kind == 1) {
+            const float ur = rng.next_1d();
+            int y = 0;
+            while (y < in.tab_h - 1 && !(in.tab[nc + size_t(y) * in.tab_w] > ur)) ++y;
+            const float uc = rng.next_1d();
+            int x = 0;
+            while (x < in.tab_w - 1 && !(in.tab[size_t(y) * in.tab_w + x] > uc)) ++x;
+            cell = y * in.tab_w + x;
+        } else {
+            const float* prob = in.tab;
+            const float* alias = in.tab + nc;
+            const int init = std::min(int(R(rng.next_1d()) * R(nc)), nc - 1);
+            const int al = std::max(0, std::min(int(R(alias[init]) + R(0.5)), nc - 1));
+            cell = R(rng.next_1d()) < R(prob[init]) ? init : al;
+        }
         const int x = cell % in.tab_w, y = cell / in.tab_w;
         const R u1 = R(rng.next_1d());
         const R u2 = R(rng.next_1d());

@@ -25,7 +25,8 @@ PTIR_STATUS = {
 EXPORTED = ["ptir_create", "ptir_destroy", "ptir_build_bvh", "ptir_trace", "ptir_trace_bwd", "ptir_tape_bytes",
             "ptir_last_error", "ptir_version", "ptir_cache_create", "ptir_cache_destroy", "ptir_cache_num_weights",
             "ptir_cache_table_floats", "ptir_cache_read_params", "ptir_cache_write_params", "ptir_cache_read_grads",
-            "ptir_cache_query", "ptir_cache_backprop", "ptir_cache_step", "ptir_cache_train", "ptir_cache_last_error"]
+            "ptir_cache_query", "ptir_cache_backprop", "ptir_cache_step", "ptir_cache_train", "ptir_cache_last_error",
+            "ptir_env_table_scratch_bytes", "ptir_env_table_build"]
 
 
 class Options(C.Structure):
@@ -40,7 +41,10 @@ class Scene(C.Structure):
                 ("particle_radiance", C.c_void_p), ("sh_stride", C.c_int32), ("shading_normal", C.c_void_p),
                 ("environment", C.c_void_p), ("env_h", C.c_int32), ("env_w", C.c_int32),
                 ("env_table", C.c_void_p), ("tab_h", C.c_int32), ("tab_w", C.c_int32),
-                ("num_lights", C.c_uint32), ("num_light_entries", C.c_uint32), ("num_mesh_lights", C.c_uint32)]
+                ("num_lights", C.c_uint32), ("num_light_entries", C.c_uint32), ("num_mesh_lights", C.c_uint32),
+                ("tab_kind", C.c_int32)]
+
+ENV_TABLE_ALIAS, ENV_TABLE_CDF = 0, 1
 
 
 class Launch(C.Structure):
@@ -119,9 +123,13 @@ def lib() -> C.CDLL:
         L.ptir_cache_train.argtypes = [vp, vp, u64, vp, vp]
         L.ptir_cache_last_error.argtypes = [vp]
         L.ptir_cache_last_error.restype = C.c_char_p
+        L.ptir_env_table_scratch_bytes.argtypes = [C.c_int32, C.c_int32]
+        L.ptir_env_table_scratch_bytes.restype = u64
+        L.ptir_env_table_build.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp, vp]
         for name in ("ptir_create", "ptir_destroy", "ptir_build_bvh", "ptir_trace", "ptir_trace_bwd",
                      "ptir_cache_create", "ptir_cache_destroy", "ptir_cache_read_params", "ptir_cache_write_params",
-                     "ptir_cache_read_grads", "ptir_cache_query", "ptir_cache_backprop", "ptir_cache_step", "ptir_cache_train"):
+                     "ptir_cache_read_grads", "ptir_cache_query", "ptir_cache_backprop", "ptir_cache_step", "ptir_cache_train",
+                     "ptir_env_table_build"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib

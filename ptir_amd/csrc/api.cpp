@@ -71,6 +71,8 @@ ptir_status check_scene(ptir_tracer* t, const ptir_scene* sc, const ptir_launch*
         return fail(t, PTIR_ERR_UNSUPPORTED, "cube-map environments ([6N,N,4]) are not supported");
     if (sc->env_table && (sc->tab_h <= 0 || sc->tab_w <= 0))
         return fail(t, PTIR_ERR_INVALID_ARGUMENT, "environment table with a non-positive size");
+    if (sc->tab_kind != PTIR_ENV_TABLE_ALIAS && sc->tab_kind != PTIR_ENV_TABLE_CDF)
+        return fail(t, PTIR_ERR_INVALID_ARGUMENT, "tab_kind must be PTIR_ENV_TABLE_ALIAS or PTIR_ENV_TABLE_CDF");
     return PTIR_OK;
 }
 
@@ -89,6 +91,7 @@ ptir::DeviceScene make_scene(ptir_tracer* t, const ptir_scene* sc, const ptir_la
     S.tab = sc->env_table;
     S.tab_w = sc->env_table ? sc->tab_w : 0;
     S.tab_h = sc->env_table ? sc->tab_h : 0;
+    S.tab_kind = sc->tab_kind;
     S.ray_to_world = la->ray_to_world;
     S.ray_ori = la->ray_ori;
     S.ray_dir = la->ray_dir;
@@ -157,6 +160,12 @@ ptir_status end_counters(ptir_tracer* t, ptir_counters* c, hipStream_t s) {
 }
 
 }  // namespace
+
+namespace ptir {
+// Error text of a failed call that has no handle (ptir_create, the environment table
+// builders); ptir_last_error(NULL) returns it.
+void set_thread_error(const std::string& msg) { g_create_error = msg; }
+}  // namespace ptir
 
 extern "C" {
 

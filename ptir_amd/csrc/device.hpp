@@ -110,6 +110,7 @@ struct DeviceScene {
     int32_t root;             // child link of the root (>=0 internal, <0 leaf ~slot), N>0
     int32_t sh_stride, sph_degree;
     int32_t env_w, env_h, tab_w, tab_h;
+    int32_t tab_kind;         // 0: alias table (prob, alias, pdf); 1: CDF table (conditional, marginal, pdf)
     int32_t kernel_degree;
     float min_response, min_transmittance;
     uint32_t max_bounces, render_opts, frame;

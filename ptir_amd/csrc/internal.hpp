@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include <cmath>
+#include <string>
 
 #include "device.hpp"
 
@@ -52,6 +53,12 @@ hipError_t prep_records(BvhBuffers& b, const KernelOpts& o, const float* density
 hipError_t scatter_material_grad(const BvhBuffers& b, float* dmat, hipStream_t s);
 // Zero a device buffer with a kernel (safe inside captured graphs, unlike hipMemsetAsync).
 hipError_t fill_zero(void* p, size_t bytes, hipStream_t s);
+
+// Environment importance table (CDF layout) built on the device; scratch of
+// env_table_scratch_bytes(h, w) bytes, 8-byte aligned.
+uint64_t env_table_scratch_bytes(int h, int w);
+hipError_t env_table_build(const float* env, int H, int W, int h, int w, float* table, void* scratch, hipStream_t s);
+void set_thread_error(const std::string& msg);
 
 hipError_t launch_trace_fwd(const DeviceScene& S, const FwdOut& out, int kbuf, bool short_stack, hipStream_t s);
 hipError_t launch_trace_bwd(const DeviceScene& S, const BwdIO& io, int kbuf, bool short_stack, hipStream_t s);

@@ -333,15 +333,36 @@ __device__ __forceinline__ V3 background_bwd(const DeviceScene& S, V3 rd, V3 cg,
     return bg;
 }
 
-// sampleEnvironmentAliasDirection (environment.cuh:227-255); draws u1..u4 in order
+// first index i in [0, n) with c[i * stride] > u (the CDFs end at exactly 1 > u)
+__device__ __forceinline__ int cdf_search(const float* c, int n, int stride, float u) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (c[size_t(mid) * stride] > u) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
+// sampleEnvironmentAliasDirection (environment.cuh:227-255); draws u1..u4 in order.  With a
+// CDF table (ptir_env_table_build) the first two draws pick the row and the column by
+// inverting the marginal and the row's conditional CDF instead.
 __device__ __forceinline__ V3 sample_env(const DeviceScene& S, Rng& rng, float& pdf) {
     const int nc = S.tab_h * S.tab_w;
-    const float* prob = S.tab;
-    const float* alias = S.tab + nc;
     const float* pdfs = S.tab + 2 * nc;
-    const int init = min(int(rng.next_1d() * float(nc)), nc - 1);
-    const int al = max(0, min(int(alias[init] + 0.5f), nc - 1));
-    const int cell = rng.next_1d() < prob[init] ? init : al;
+    int cell;
+    if (S.tab_kind == 1) {
+        const float ur = rng.next_1d();
+        const int yr = cdf_search(S.tab + nc, S.tab_h, S.tab_w, ur);
+        const float uc = rng.next_1d();
+        cell = yr * S.tab_w + cdf_search(S.tab + size_t(yr) * S.tab_w, S.tab_w, 1, uc);
+    } else {
+        const float* prob = S.tab;
+        const float* alias = S.tab + nc;
+        const int init = min(int(rng.next_1d() * float(nc)), nc - 1);
+        const int al = max(0, min(int(alias[init] + 0.5f), nc - 1));
+        cell = rng.next_1d() < prob[init] ? init : al;
+    }
     const int x = cell % S.tab_w, y = cell / S.tab_w;
     const float u1 = rng.next_1d();
     const float u2 = rng.next_1d();

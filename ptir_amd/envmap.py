@@ -1,106 +1,137 @@
 """Environment importance tables consumed by the tracer.
 
-`build_environment_alias_table` restates the reference's caller-side table build
-(threedgrut/model/aliastable.py:49-116,244-297): the env is area-resampled to 64x128,
-weighted by Rec.709 luminance times the equirect texel solid angle, and turned into a
-Vose alias table (FP64 weights, FP32 storage) plus the per-steradian pdf
-``w / (sum(w) * solid_angle)``.  `pack_alias_table` packs it as the ``[3, h, w]``
-(prob, alias-as-float, pdf) tensor the tracer reads (threedgptir_tracer/tracer.py:819-847).
+Two layouts of the ``[3, h, w]`` float32 table the tracer reads (``ptir_scene.tab_kind``):
 
-The table is a caller-side input of the tracer, built every 32 training steps in the
-reference; it is not on the per-path hot loop.
+* **alias** -- the reference's: the caller area-resamples the environment to 64x128,
+  weights each cell by Rec.709 luminance times its equirect solid angle and builds a
+  Vose alias table on the host (threedgrut/model/aliastable.py:49-116,244-297; packed by
+  threedgptir_tracer/tracer.py:819-847 as (prob, alias-as-float, pdf)).
+  :func:`build_environment_alias_table` restates that build so the tracer can be fed
+  exactly the cells the reference would sample.
+* **cdf** -- built on the GPU by :func:`build_cdf_table` (``ptir_env_table_build``):
+  the same weights and per-steradian pdf, inverted through a marginal CDF over rows and a
+  conditional CDF per row, with FP64 prefix sums.  No host round trip, so it can be
+  rebuilt every training step (the reference rebuilds the alias table every 32 steps).
 """
 
 from __future__ import annotations
 
+import ctypes as C
 from dataclasses import dataclass
 
 import numpy as np
 import torch
 import torch.nn.functional as F
 
-DEFAULT_ALIAS_TABLE_SIZE = (64, 128)
-LUMINANCE = (0.2126, 0.7152, 0.0722)
+TABLE_GRID = (64, 128)  # rows (elevation) x columns (azimuth), aliastable.py:24
+REC709 = (0.2126, 0.7152, 0.0722)
 
 
 @dataclass(frozen=True)
-class EnvAliasTable:
-    width: int
+class AliasTable:
+    """Per-cell arrays of an alias table over an h x w equirect grid (row-major)."""
+
+    prob: torch.Tensor   # [h*w] float32 acceptance probability of the cell itself
+    alias: torch.Tensor  # [h*w] int32 cell taken on rejection
+    pdf: torch.Tensor    # [h*w] float32 density per steradian
     height: int
-    numCells: int
-    prob: torch.Tensor
-    alias: torch.Tensor
-    pdf: torch.Tensor
+    width: int
+
+    @property
+    def num_cells(self) -> int:
+        return self.height * self.width
 
 
-def build_alias_table(weights) -> tuple[torch.Tensor, torch.Tensor]:
-    """Vose alias table from non-negative weights (aliastable.py:49-116).
+def vose_alias(weights) -> tuple[torch.Tensor, torch.Tensor]:
+    """(prob, alias) of Vose's method over non-negative weights, in float64 arithmetic.
 
-    Small/large work lists are popped from the back, exactly as the reference does, so
-    the (prob, alias) pairing is the reference's."""
-    w = torch.as_tensor(weights).detach().reshape(-1).to("cpu", torch.float64)
-    w = torch.where(torch.isfinite(w) & (w > 0), w, torch.zeros_like(w))
-    n = w.numel()
-    if n == 0:
-        raise ValueError("weights must contain at least one element.")
-    total = float(w.sum())
+    The under-full and over-full work lists are consumed from their ends, which fixes the
+    (prob, alias) pairing to the one aliastable.py:49-116 produces; the sampled cells of the
+    tracer depend on that pairing."""
+    w = torch.as_tensor(weights).detach().reshape(-1).to("cpu", torch.float64).numpy()
+    if w.size == 0:
+        raise ValueError("an alias table needs at least one weight")
+    w = np.where(np.isfinite(w) & (w > 0), w, 0.0)
+    n = w.size
+    total = w.sum()
     if total <= 0.0:
         return torch.ones(n, dtype=torch.float32), torch.arange(n, dtype=torch.int32)
-    scaled = w.numpy() * (float(n) / total)
-    prob = np.empty(n, dtype=np.float32)
+    level = w * (n / total)  # mean 1
+    prob = np.ones(n, dtype=np.float32)
     alias = np.arange(n, dtype=np.int64)
-    small = [int(i) for i in np.nonzero(scaled < 1.0)[0]]
-    large = [int(i) for i in np.nonzero(scaled >= 1.0)[0]]
-    while small and large:
-        s = small.pop()
-        l = large.pop()
-        prob[s] = np.float32(np.clip(scaled[s], 0.0, 1.0))
-        alias[s] = l
-        scaled[l] -= 1.0 - scaled[s]
-        (small if scaled[l] < 1.0 else large).append(l)
-    for i in small + large:
-        prob[i] = 1.0
-        alias[i] = i
+    under = list(np.flatnonzero(level < 1.0))
+    over = list(np.flatnonzero(level >= 1.0))
+    while under and over:
+        lo, hi = under.pop(), over.pop()
+        prob[lo] = np.float32(min(max(level[lo], 0.0), 1.0))
+        alias[lo] = hi
+        level[hi] -= 1.0 - level[lo]
+        if level[hi] < 1.0:
+            under.append(hi)
+        else:
+            over.append(hi)
+    # whatever is left holds (up to rounding) exactly one unit of mass: keep it
     return torch.from_numpy(prob), torch.from_numpy(alias.astype(np.int32))
 
 
-def equirect_solid_angles(h: int, w: int, dtype=torch.float32) -> torch.Tensor:
-    """Solid angle of each equirect texel (aliastable.py:163-176)."""
-    edges = (torch.arange(h + 1, dtype=dtype) / float(h) - 0.5) * torch.pi
-    rows = (2.0 * torch.pi / float(w)) * (torch.sin(edges[1:]) - torch.sin(edges[:-1]))
-    return torch.clamp(rows, min=0.0).reshape(h, 1).expand(h, w).contiguous()
+def cell_solid_angles(h: int, w: int, dtype=torch.float32) -> torch.Tensor:
+    """[h, w] solid angle of each equirect cell: (2 pi / w)(sin(lat_top) - sin(lat_bottom))."""
+    lat = (torch.arange(h + 1, dtype=dtype) / h - 0.5) * torch.pi
+    band = (2.0 * torch.pi / w) * torch.diff(torch.sin(lat))
+    return band.clamp_min(0.0)[:, None].expand(h, w).contiguous()
 
 
-def build_environment_alias_table(environment: torch.Tensor, target_size=DEFAULT_ALIAS_TABLE_SIZE) -> EnvAliasTable:
+def _cell_weights(environment: torch.Tensor, grid) -> tuple[torch.Tensor, torch.Tensor]:
     env = torch.as_tensor(environment).detach().cpu()
-    if env.ndim != 3 or env.shape[-1] < 3:
-        raise ValueError(f"Environment must have shape [H, W, C>=3], got {tuple(env.shape)}")
-    th, tw = int(target_size[0]), int(target_size[1])
-    if tuple(env.shape[:2]) != (th, tw):
-        env = F.interpolate(env.float().permute(2, 0, 1).unsqueeze(0), size=(th, tw), mode="area")
-        env = env.squeeze(0).permute(1, 2, 0).contiguous()
+    if env.dim() != 3 or env.shape[-1] < 3:
+        raise ValueError(f"environment must be [H, W, C] with C >= 3, not {tuple(env.shape)}")
+    h, w = int(grid[0]), int(grid[1])
+    if tuple(env.shape[:2]) != (h, w):
+        env = F.interpolate(env.float().permute(2, 0, 1)[None], size=(h, w), mode="area")[0].permute(1, 2, 0)
     rgb = env[..., :3].float()
-    rgb = torch.where(torch.isfinite(rgb) & (rgb > 0), rgb, torch.zeros_like(rgb))
-    lum = torch.sum(rgb * rgb.new_tensor(LUMINANCE), dim=-1)
-    h, w = lum.shape
-    sa = equirect_solid_angles(h, w, lum.dtype)
-    sw = (lum * sa).contiguous()
-    total = sw.sum()
+    rgb = torch.where(torch.isfinite(rgb) & (rgb > 0), rgb, 0.0)
+    lum = (rgb * rgb.new_tensor(REC709)).sum(-1)
+    sa = cell_solid_angles(h, w, lum.dtype)
+    return (lum * sa).contiguous(), sa
+
+
+def build_environment_alias_table(environment: torch.Tensor, target_size=TABLE_GRID) -> AliasTable:
+    """Alias table of the reference's caller-side build (aliastable.py:244-297)."""
+    wts, sa = _cell_weights(environment, target_size)
+    total = wts.sum()
     if not bool(torch.isfinite(total)) or float(total) <= 0.0:
-        sw = sa
-        total = sw.sum()
-    pdf = torch.where(sa > 0, sw / torch.clamp(total * sa, min=torch.finfo(sw.dtype).tiny), torch.zeros_like(sw))
-    prob, alias = build_alias_table(sw)
-    return EnvAliasTable(width=w, height=h, numCells=h * w, prob=prob, alias=alias,
-                         pdf=pdf.reshape(-1).float().contiguous())
+        wts, total = sa, sa.sum()  # no usable light: sample by solid angle
+    pdf = torch.where(sa > 0, wts / torch.clamp(total * sa, min=torch.finfo(wts.dtype).tiny), 0.0)
+    prob, alias = vose_alias(wts)
+    h, w = wts.shape
+    return AliasTable(prob=prob, alias=alias, pdf=pdf.reshape(-1).float().contiguous(), height=h, width=w)
 
 
-def pack_alias_table(table: EnvAliasTable | None, device="cpu") -> torch.Tensor:
-    """[3, h, w] float32 = (prob, alias, pdf), or an empty tensor (tracer.py:824-847)."""
+def pack_alias_table(table: AliasTable | None, device="cpu") -> torch.Tensor:
+    """The tracer's [3, h, w] alias layout (prob, alias as float, pdf), or an empty tensor."""
     if table is None:
         return torch.empty(0, dtype=torch.float32, device=device)
-    return torch.cat([
-        table.prob.reshape(1, table.height, table.width),
-        table.alias.reshape(1, table.height, table.width).to(torch.float32),
-        table.pdf.reshape(1, table.height, table.width),
-    ], dim=0).to(device).contiguous()
+    planes = [table.prob, table.alias.to(torch.float32), table.pdf]
+    return torch.stack([p.reshape(table.height, table.width).float() for p in planes]).to(device).contiguous()
+
+
+def build_cdf_table(environment: torch.Tensor, grid=TABLE_GRID, out: torch.Tensor | None = None) -> torch.Tensor:
+    """[3, h, w] CDF-layout table of a device environment [H, W, 4], built on the GPU on the
+    current stream (ptir_env_table_build)."""
+    from . import _lib
+
+    env = environment
+    if not (isinstance(env, torch.Tensor) and env.is_cuda and env.dtype == torch.float32):
+        raise TypeError("environment must be a float32 device tensor")
+    if env.dim() != 3 or env.shape[-1] != 4:
+        raise ValueError(f"environment must be [H, W, 4], not {tuple(env.shape)}")
+    env = env.contiguous()
+    h, w = int(grid[0]), int(grid[1])
+    if out is None:
+        out = torch.empty(3, h, w, dtype=torch.float32, device=env.device)
+    elif tuple(out.shape) != (3, h, w) or out.dtype != torch.float32 or not out.is_contiguous():
+        raise ValueError("out must be a contiguous float32 [3, h, w] tensor")
+    stream = torch.cuda.current_stream(env.device).cuda_stream
+    _lib.check(_lib.lib().ptir_env_table_build(env.data_ptr(), env.shape[0], env.shape[1], h, w, out.data_ptr(),
+                                               C.c_void_p(stream)), None)
+    return out

@@ -57,7 +57,8 @@ class HipTracer:
                  particle_kernel_density_clamping: bool = True, particle_radiance_sph_degree: int = 3,
                  enable_normals: bool = True, enable_hitcounts: bool = True, enable_mis: bool = True,
                  enable_metallic: bool = False, visualize_lights: bool = False, kbuffer: int = 16,
-                 use_tape: bool = True, tape_hits_per_path: int = 64, traversal: str = "wave"):
+                 use_tape: bool = True, tape_hits_per_path: int = 64, traversal: str = "wave",
+                 env_sampling: str = "alias"):
         if pipeline != "reference" or backward_pipeline not in ("referenceBwd", "reference" + "Bwd"):
             raise ValueError(f"unsupported pipeline pair ({pipeline}, {backward_pipeline}); "
                              "only the PTIR 'reference' path tracer is built")
@@ -65,6 +66,11 @@ class HipTracer:
             raise ValueError(f"unsupported primitive type {primitive!r}; only 'instances' is built")
         if traversal not in ("lane", "wave"):
             raise ValueError(f"traversal must be 'lane' or 'wave', not {traversal!r}")
+        if env_sampling not in ("alias", "cdf"):
+            raise ValueError(f"env_sampling must be 'alias' (caller-built alias table) or 'cdf' "
+                             f"(envmap.build_cdf_table), not {env_sampling!r}")
+        # layout of the environment table passed to trace / trace_bwd
+        self.tab_kind = _lib.ENV_TABLE_ALIAS if env_sampling == "alias" else _lib.ENV_TABLE_CDF
         o = _lib.Options(float(particle_kernel_degree), float(particle_kernel_min_response),
                          int(bool(particle_kernel_density_clamping)), int(particle_radiance_sph_degree),
                          int(bool(enable_normals)), int(bool(enable_hitcounts)), int(bool(enable_mis)),
@@ -147,6 +153,7 @@ class HipTracer:
             tab = _need(tab, "environment_alias_table")
             keep.append(tab)
             sc.env_table, sc.tab_h, sc.tab_w = _ptr(tab), tab.shape[1], tab.shape[2]
+        sc.tab_kind = self.tab_kind
         sc.num_lights = 0 if lights is None else lights.shape[0]
         sc.num_light_entries = 0 if light_alias_table is None or light_alias_table.dim() < 2 else light_alias_table.shape[1]
         sc.num_mesh_lights = 0 if mesh_lights is None else mesh_lights.shape[0]

@@ -221,7 +221,7 @@ def test_env_bilinear_partition_of_unity_and_texel_centres():
 def test_alias_table_reproduces_weights():
     w = torch.tensor(np.random.default_rng(4).lognormal(size=777))
     w[::13] = 0
-    prob, alias = envmap.build_alias_table(w)
+    prob, alias = envmap.vose_alias(w)
     n = w.numel()
     p = prob.double().numpy()
     a = alias.numpy()
@@ -234,7 +234,7 @@ def test_env_table_pdf_is_a_density_over_the_sphere():
     g = torch.Generator().manual_seed(0)
     env = scenes.lognormal_env(32, 64, g)
     t = envmap.build_environment_alias_table(env)
-    sa = envmap.equirect_solid_angles(t.height, t.width, torch.float64).reshape(-1)
+    sa = envmap.cell_solid_angles(t.height, t.width, torch.float64).reshape(-1)
     assert float((t.pdf.double() * sa).sum()) == pytest.approx(1.0, rel=1e-5)
     assert abs(float(sa.sum()) - 4 * math.pi) < 1e-9
 
@@ -251,14 +251,14 @@ def test_env_alias_sampling_histogram_matches_pdf():
     sub = dirs[:2000]
     np.testing.assert_allclose([O.env_pdf(inp, d) for d in sub], pdfs[:2000], rtol=1e-9)
     # histogram over cells vs P(cell) = pdf * solid angle
-    sa = envmap.equirect_solid_angles(t.height, t.width, torch.float64).reshape(-1).numpy()
+    sa = envmap.cell_solid_angles(t.height, t.width, torch.float64).reshape(-1).numpy()
     p_cell = t.pdf.double().numpy() * sa
     # cell index from the reported pdf is ambiguous; re-derive cell from direction
     e = np.stack([-dirs[:, 1], dirs[:, 0], dirs[:, 2]], -1)  # env rotation by pi/2 about z
     u = (np.arctan2(e[:, 0], e[:, 1]) + np.pi) / (2 * np.pi)
     v = 0.5 + np.arcsin(np.clip(-e[:, 2], -1, 1)) / np.pi
     cell = np.clip((v * t.height).astype(int), 0, t.height - 1) * t.width + np.clip((u * t.width).astype(int), 0, t.width - 1)
-    hist = np.bincount(cell, minlength=t.numCells) / n
+    hist = np.bincount(cell, minlength=t.num_cells) / n
     big = p_cell > 20.0 / n
     chi = ((hist[big] - p_cell[big]) ** 2 / p_cell[big]).sum() * n
     dof = big.sum() - 1
