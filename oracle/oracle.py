@@ -47,7 +47,7 @@ _FWD_CH = {"rad": 3, "dns": 1, "hit": 2, "hit2": 1, "distortion": 1, "nrm": 3, "
 
 
 class OracleFwdOut(C.Structure):
-    _fields_ = [(k, C.c_void_p) for k in _FWD_KEYS]
+    _fields_ = [(k, C.c_void_p) for k in _FWD_KEYS] + [("margin", C.c_void_p)]
 
 
 class OracleBwdIn(C.Structure):
@@ -173,7 +173,8 @@ def trace(inp: Inputs, rays: Rays, precision=64, nthreads=None, counters=False):
     B, H, W = rays.shape
     out = {k: np.zeros((B, H, W, c), dtype=np.float64) for k, c in _FWD_CH.items()}
     out["visibility"] = np.zeros(inp.s.n, dtype=np.float64)
-    o = OracleFwdOut(**{k: _ptr(out[k]) for k in _FWD_KEYS})
+    out["margin"] = np.ones((B, H, W), dtype=np.float64)
+    o = OracleFwdOut(**{k: _ptr(out[k]) for k in _FWD_KEYS + ["margin"]})
     cnt = OracleCounters()
     rc = lib().oracle_trace(C.byref(inp.s), C.byref(rays.r), C.byref(o), precision, nthreads or os.cpu_count(),
                             C.byref(cnt))

@@ -185,6 +185,12 @@ struct OracleRays {
 struct OracleFwdOut {
     double *rad, *dns, *hit, *hit2, *distortion, *nrm, *snrm, *mat, *hits, *pbr, *light, *comp;
     double* visibility;  // [n]
+    // [B,H,W] (optional): smallest relative distance of any discrete decision of the path to
+    // its threshold (hit acceptance, candidate box and distance tests, transmittance cut-off,
+    // interaction opacity, escape and shadow draws, self-occlusion, hemisphere tests, the
+    // throughput cut-off, nearest env-table cell).  A path whose margin is well above float32
+    // rounding takes the same decisions in any float32 evaluation of the same equations.
+    double* margin;
 };
 
 struct OracleBwdIn {
@@ -403,8 +409,18 @@ static void build_scene(Scene<R>& sc, const OracleScene* in) {
 }
 
 // Candidate test for Gaussian i (see header): peak t*, inside-cube and distance tests.
+// Margin bookkeeping: m = min(m, |a - b| / max(|a|, |b|)) (relative distance to a threshold)
+static inline void note_rel(double* m, double a, double b) {
+    if (!m) return;
+    const double s = std::max(std::fabs(a), std::fabs(b));
+    if (s > 0) *m = std::min(*m, std::fabs(a - b) / s);
+}
+static inline void note_abs(double* m, double d) {
+    if (m) *m = std::min(*m, std::fabs(d));
+}
+
 template <class R>
-static bool candidate(const Gauss<R>& G, const Ray<R>& ray, R& tstar) {
+static bool candidate(const Gauss<R>& G, const Ray<R>& ray, R& tstar, double* margin = nullptr) {
     const V3<R> dp = ray.o - G.mu;
     // object space of M = R diag(kscl):  x' = diag(1/kscl) R^T x
     const V3<R> rtp = v3<R>(G.rot[0][0] * dp.x + G.rot[1][0] * dp.y + G.rot[2][0] * dp.z,
@@ -419,14 +435,18 @@ static bool candidate(const Gauss<R>& G, const Ray<R>& ray, R& tstar) {
     if (!(dd > 0)) return false;
     tstar = -dot(op, dpr) / dd;
     const V3<R> foot = op + dpr * tstar;
-    if (std::fabs(foot.x) > 1 || std::fabs(foot.y) > 1 || std::fabs(foot.z) > 1) return false;
+    const R fmax_ = std::max(std::fabs(foot.x), std::max(std::fabs(foot.y), std::fabs(foot.z)));
+    note_abs(margin, double(fmax_) - 1.0);
+    if (fmax_ > 1) return false;
+    note_abs(margin, double(tstar));
     const V3<R> gc = cross(safe_normalize(dpr), op);
+    note_rel(margin, double(dot(gc, gc) / dd), 9.0);
     return dot(gc, gc) / dd < R(9);  // gaussianParticles.cuh:496-497
 }
 
 // All candidates along the ray with t* > 0, sorted by (t*, id).
 template <class R>
-static void collect(const Scene<R>& sc, const Ray<R>& ray, std::vector<Cand>& out) {
+static void collect(const Scene<R>& sc, const Ray<R>& ray, std::vector<Cand>& out, double* margin = nullptr) {
     out.clear();
     if (sc.nodes.empty()) return;
     const V3<R> inv = v3<R>(R(1) / ray.d.x, R(1) / ray.d.y, R(1) / ray.d.z);
@@ -443,7 +463,7 @@ static void collect(const Scene<R>& sc, const Ray<R>& ray, std::vector<Cand>& ou
             for (int k = nd.first; k < nd.first + nd.count; ++k) {
                 const uint32_t id = sc.order[k];
                 R t;
-                if (candidate(sc.g[id], ray, t) && t > 0) out.push_back({double(t), id});
+                if (candidate(sc.g[id], ray, t, margin) && t > 0) out.push_back({double(t), id});
             }
         } else {
             stack[sp++] = nd.left;
@@ -480,6 +500,10 @@ struct Tracer {
     double* dmat = nullptr;
     double* denv = nullptr;
     double* visibility = nullptr;
+    // decision margin of the path being traced (see OracleFwdOut::margin)
+    mutable double margin = 1.0;
+    void note(double a, double b) const { note_rel(&margin, a, b); }
+    void note0(double d) const { note_abs(&margin, d); }
 
     explicit Tracer(const Scene<R>& s) : sc(s), in(*s.in) {}
 
@@ -488,8 +512,12 @@ struct Tracer {
     // pathTracer.cuh:115-128
     bool self_occluded(uint32_t id, R t, const Ray<R>& ray) const {
         const float* sn = sn_of(id);
-        if (t > R(kMaxSelfOcc) || !sn) return false;
-        return dot(v3<R>(sn[0], sn[1], sn[2]), ray.d) > R(kSelfOccDot);
+        if (!sn) return false;
+        const R dn = dot(v3<R>(sn[0], sn[1], sn[2]), ray.d);
+        if (dn > R(kSelfOccDot)) note(double(t), kMaxSelfOcc);
+        if (t > R(kMaxSelfOcc)) return false;
+        note(double(dn), kSelfOccDot);
+        return dn > R(kSelfOccDot);
     }
 
     // gaussianParticles.cuh:343-440 (alpha part), also pathTracer.cuh:130-159, 350-374
@@ -509,6 +537,8 @@ struct Tracer {
         const R gres = response<R>(int(in.kernel_degree), gd);
         const R a = std::min(R(0.99), gres * G.sigma);
         if (gro_o) { *gro_o = gro; *grd_o = grd; *gd_o = gd; }
+        note(double(gres), double(in.min_response));
+        note(double(gres * G.sigma), 1.0 / 255.0);
         return (gres > R(in.min_response) && a > R(1.0 / 255.0)) ? a : R(0);
     }
 
@@ -556,13 +586,14 @@ struct Tracer {
         cnt.segments++;
         bool gathered = false;
         while (pl.lastHit <= t1 && pl.T > minT) {
-            if (!gathered) { collect(sc, ray, cands); gathered = true; }
+            if (!gathered) { collect(sc, ray, cands, &margin); gathered = true; }
             const double lo = double(adv(pl.lastHit)), hi = double(adv(t1));
             size_t b = chunk_begin(cands, lo);
             size_t e = b;
             while (e < cands.size() && int(e - b) < in.kbuf && cands[e].t < hi) ++e;
             if (b == e) break;
             for (size_t k = b; k < e; ++k) {
+                note(double(pl.T), double(minT));
                 if (!(pl.T > minT)) continue;
                 const uint32_t id = cands[k].id;
                 const R t = R(cands[k].t);
@@ -612,6 +643,7 @@ struct Tracer {
         const R op = R(1) - pl.T;
         pl.hitDist = depth;
         pl.valid = false;
+        if (pl.hit) note(double(op), 0.5);
         if (pl.hit && op > R(0.5)) {
             // interaction.h:47-68
             const R inv = R(1) / std::max(op, R(1e-12));
@@ -625,6 +657,7 @@ struct Tracer {
             pl.inter.g = Mat<R>();
         } else {
             const R u = R(rng.next_1d());
+            note(double(u), double(pl.T));
             pl.valid = u < pl.T;
         }
     }
@@ -638,7 +671,7 @@ struct Tracer {
         if (startT >= endT) return false;
         bool gathered = false;
         while (startT < endT) {
-            if (!gathered) { collect(sc, ray, cands); gathered = true; }
+            if (!gathered) { collect(sc, ray, cands, &margin); gathered = true; }
             size_t b = chunk_begin(cands, double(adv(startT)));
             size_t e = b;
             while (e < cands.size() && int(e - b) < in.kbuf && cands[e].t < double(endT)) ++e;
@@ -652,7 +685,11 @@ struct Tracer {
                 cnt.shadow_candidates++;
                 if (self_occluded(id, t, ray)) continue;
                 const R a = hit_alpha(id, ray);
-                if (a > 0 && R(rng.next_1d()) < a) return true;
+                if (a > 0) {
+                    const R u = R(rng.next_1d());
+                    note(double(u), double(a));
+                    if (u < a) return true;
+                }
             }
             if (batchEnd <= startT) break;
             startT = batchEnd;
@@ -793,6 +830,8 @@ struct Tracer {
         const V3<R> dir = env_rotate(rd);
         R u, v;
         equirect_uv(dir, u, v);
+        const double xu = double(u) * in.tab_w, yv = double(v) * in.tab_h;
+        note0(std::min(std::fabs(xu - std::round(xu)) / in.tab_w, std::fabs(yv - std::round(yv)) / in.tab_h));
         int x = int(u * R(in.tab_w));
         x = x % in.tab_w;
         if (x < 0) x += in.tab_w;
@@ -928,6 +967,8 @@ struct Tracer {
                 }
             }
         }
+        note0(double(dot(n, L)));
+        note0(double(dot(n, wo)));
         if (dot(n, L) <= 0) return res;
         R p;
         const BG bc = eval_ls(wo, n, albedo, metal, roughness, L, p, grads);
@@ -974,6 +1015,7 @@ struct Tracer {
         const V3<R> Li = background(wi);
         const R lightPdf = envPdf;
         if (lightPdf <= 0 || is_zero(Li)) { p.emit = Seg<R>(); return; }
+        note0(double(dot(safe_normalize(it.sn), wi)));
         if (dot(it.sn, wi) <= 0) { p.emit = Seg<R>(); return; }
         V3<R> n, wo, alb;
         R met;
@@ -1117,7 +1159,7 @@ struct Tracer {
         const R minT = R(in.min_transmittance);
         bool gathered = false;
         while (startT < endT && T > minT) {
-            if (!gathered) { collect(sc, ray, cands); gathered = true; }
+            if (!gathered) { collect(sc, ray, cands, &margin); gathered = true; }
             size_t b = chunk_begin(cands, double(adv(startT)));
             size_t e = b;
             while (e < cands.size() && int(e - b) < in.kbuf && cands[e].t < double(endT)) ++e;
@@ -1252,6 +1294,7 @@ struct Tracer {
         Path<R> p;
         p.maxBounces = in.max_bounces;
         cnt.paths++;
+        margin = 1.0;
         ray_intersect(primary_ray(r, pix), p.cur, rng, false, true);
         const Seg<R>& c = p.cur;
         auto put = [&](double* buf, int C, int k, R v) { if (buf) buf[pix * C + k] = double(v); };
@@ -1267,7 +1310,10 @@ struct Tracer {
         put(out.mat, 5, 0, c.inter.m.albedo.x); put(out.mat, 5, 1, c.inter.m.albedo.y); put(out.mat, 5, 2, c.inter.m.albedo.z);
         put(out.mat, 5, 3, c.inter.m.rough); put(out.mat, 5, 4, c.inter.m.metal);
         if (in.enable_hitcounts) put(out.hits, 1, 0, c.hits);
-        if (!p.cur.inter.valid) return;
+        if (!p.cur.inter.valid) {
+            if (out.margin) out.margin[pix] = margin;
+            return;
+        }
         if (in.enable_mis) sample_nee(p, rng);
         for (unsigned depth = 0; depth < in.max_bounces && p.active; ++depth) {
             accumulate(p);
@@ -1275,9 +1321,11 @@ struct Tracer {
             if (!p.active) break;
             p.numBounces = depth + 1u;
             sample_next(p, rng);
+            if (!is_zero(p.thr)) note(double(vmax(p.thr)), 1e-4);
             if (vmax(p.thr) < R(1e-4)) break;
             ray_intersect(p.cur.ray, p.cur, rng, true, false);
         }
+        if (out.margin) out.margin[pix] = margin;
         put(out.pbr, 3, 0, p.L.x); put(out.pbr, 3, 1, p.L.y); put(out.pbr, 3, 2, p.L.z);
         put(out.light, 3, 0, p.Lnb.x); put(out.light, 3, 1, p.Lnb.y); put(out.light, 3, 2, p.Lnb.z);
         put(out.comp, 6, 0, p.direct.x); put(out.comp, 6, 1, p.direct.y); put(out.comp, 6, 2, p.direct.z);

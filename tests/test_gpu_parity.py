@@ -43,15 +43,56 @@ def test_forward_gbuffer_parity(small_run):
     _, _, _, g, _, _, ref = small_run
     print({k: pixel_agreement(g[k], ref[k], RTOL, ATOL) for k in ("rad", "dns", "hit", "mat", "pbr")}, flush=True)
     for k in ("rad", "dns", "hit", "hit2", "distortion", "nrm", "snrm", "mat", "hits"):
-        assert pixel_agreement(g[k], ref[k], RTOL, ATOL) > 0.995, k
+        assert pixel_agreement(g[k], ref[k], RTOL, ATOL) >= 0.999, k
 
 
 def test_forward_pbr_parity(small_run):
     _, _, _, g, _, _, ref = small_run
     for k in ("pbr", "light", "comp"):
-        assert pixel_agreement(g[k], ref[k], RTOL, ATOL) > 0.98, k
+        assert pixel_agreement(g[k], ref[k], RTOL, ATOL) >= 0.999, k
         m_g, m_r = g[k].mean(), ref[k].mean()
         assert abs(m_g - m_r) <= 1e-2 * abs(m_r), k
+
+
+MARGIN_FRAGILE = 1e-2  # decisions this close (relative) to a threshold may flip in float32
+
+
+def _disagreement_report(g, ref64, ref32, keys=("dns", "hit", "mat", "pbr", "light", "comp")):
+    """Per path: does the GPU agree with the FP64 oracle, and the FP32 oracle with the FP64
+    one; and the FP64 oracle's decision margin of each path (the smallest relative distance
+    of any discrete decision to its threshold)."""
+    def agree(a, b):
+        ok = np.ones(a["pbr"].shape[:3], bool)
+        for k in keys:
+            x = a[k].reshape(*a[k].shape[:3], -1)
+            y = b[k].reshape(x.shape)
+            ok &= (np.abs(x - y) <= ATOL + RTOL * np.abs(y)).all(-1)
+        return ok
+    g64, o3264 = agree(g, ref64), agree(ref32, ref64)
+    m = ref64["margin"]
+    fragile = m < MARGIN_FRAGILE
+    return {"paths": int(g64.size), "gpu_vs_fp64_differ": int((~g64).sum()),
+            "fp32_oracle_vs_fp64_differ": int((~o3264).sum()), "fragile_paths": int(fragile.sum()),
+            "gpu_differ_not_fragile": int((~g64 & ~fragile).sum()),
+            "max_margin_of_gpu_differ": float(m[~g64].max()) if (~g64).any() else 0.0}
+
+
+def test_disagreeing_paths_are_float32_rounding(small, small_run):
+    """The paths on which the FP32 GPU result differs from the FP64 oracle are (a) no more
+    numerous than those on which the oracle's own FP32 run differs from its FP64 run, and
+    (b) all within float32 reach of a discrete decision: their FP64 decision margin is
+    below 1e-2, a property a third of the paths do not have.  (The FP32 oracle does not
+    reproduce the GPU's flips path by path: its arithmetic, e.g. per-use quaternion
+    matrices, rounds differently from the GPU's precomputed records.)"""
+    sc, tab = small
+    _, _, _, g, _, _, ref = small_run
+    _, _, ref32 = oracle_forward(sc, tab, precision=32)
+    rep = _disagreement_report(g, ref, ref32)
+    print("small-scene disagreement", rep, flush=True)
+    assert rep["gpu_vs_fp64_differ"] <= 1e-3 * rep["paths"]
+    assert rep["gpu_vs_fp64_differ"] <= 2 * rep["fp32_oracle_vs_fp64_differ"] + 5
+    assert rep["gpu_differ_not_fragile"] == 0
+    assert rep["fragile_paths"] <= 0.8 * rep["paths"]
 
 
 def test_visibility_matches(small_run):
@@ -192,6 +233,13 @@ def test_object_scene_crop_matches_oracle(device):
         assert pixel_agreement(g[k], ref[k], RTOL, ATOL) > 0.97, k
     ok = _agreeing_paths(g, ref)
     assert ok.mean() > 0.97
+    _, _, ref32 = oracle_forward(crop, tab, precision=32)
+    rep = _disagreement_report(g, ref, ref32)
+    print("object-crop disagreement", rep, flush=True)
+    # the dense shell puts every path within 1e-2 of some decision, so here the check is the
+    # rate: the GPU differs from FP64 no more often than the FP32 oracle does
+    assert rep["gpu_vs_fp64_differ"] <= 1.5 * rep["fp32_oracle_vs_fp64_differ"] + 10
+    assert rep["gpu_differ_not_fragile"] == 0
     mask = torch.from_numpy(ok.astype(np.float32)).to(device).unsqueeze(-1)
     gpbr = (torch.full_like(outs[10], 1e-3) * mask).contiguous()
     gr = gpu_backward(tr, crop, outs, args, gpbr)
