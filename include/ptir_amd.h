@@ -265,6 +265,16 @@ uint64_t ptir_env_table_scratch_bytes(int32_t tab_h, int32_t tab_w);
 ptir_status ptir_env_table_build(const float* environment, int32_t env_h, int32_t env_w, int32_t tab_h, int32_t tab_w,
                                  float* table, void* stream);
 
+/* ptir_env_table_build without any allocation: no counterpart in the reference.  For callers
+ * that cannot allocate device memory (code under HIP graph capture, serving loops, harnesses
+ * that forbid hipMalloc).  Its scratch is a static device array of the library (1.08 MB per
+ * GPU, held until the process ends), so tables are limited to 131072 cells (e.g. 256 x 512)
+ * and 4096 rows; larger ones return PTIR_ERR_INVALID_ARGUMENT.  Same kernels and results as
+ * ptir_env_table_build.  Every call on one GPU shares that array: calls that overlap on one
+ * GPU (different streams or threads) race and must be serialised by the caller. */
+ptir_status ptir_env_table_build_noalloc(const float* environment, int32_t env_h, int32_t env_w, int32_t tab_h,
+                                         int32_t tab_w, float* table, void* stream);
+
 /* Human-readable text of the last error on this handle (or, when tracer is NULL, of the
  * last failed call without a handle on this thread: ptir_create, the table builders).  Valid until the next call on the handle. */
 const char* ptir_last_error(const ptir_tracer* tracer);

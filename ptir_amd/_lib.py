@@ -26,7 +26,7 @@ EXPORTED = ["ptir_create", "ptir_destroy", "ptir_build_bvh", "ptir_trace", "ptir
             "ptir_last_error", "ptir_version", "ptir_cache_create", "ptir_cache_destroy", "ptir_cache_num_weights",
             "ptir_cache_table_floats", "ptir_cache_read_params", "ptir_cache_write_params", "ptir_cache_read_grads",
             "ptir_cache_query", "ptir_cache_backprop", "ptir_cache_step", "ptir_cache_train", "ptir_cache_last_error",
-            "ptir_env_table_scratch_bytes", "ptir_env_table_build"]
+            "ptir_env_table_scratch_bytes", "ptir_env_table_build", "ptir_env_table_build_noalloc"]
 
 
 class Options(C.Structure):
@@ -126,10 +126,11 @@ def lib() -> C.CDLL:
         L.ptir_env_table_scratch_bytes.argtypes = [C.c_int32, C.c_int32]
         L.ptir_env_table_scratch_bytes.restype = u64
         L.ptir_env_table_build.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp, vp]
+        L.ptir_env_table_build_noalloc.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp, vp]
         for name in ("ptir_create", "ptir_destroy", "ptir_build_bvh", "ptir_trace", "ptir_trace_bwd",
                      "ptir_cache_create", "ptir_cache_destroy", "ptir_cache_read_params", "ptir_cache_write_params",
                      "ptir_cache_read_grads", "ptir_cache_query", "ptir_cache_backprop", "ptir_cache_step", "ptir_cache_train",
-                     "ptir_env_table_build"):
+                     "ptir_env_table_build", "ptir_env_table_build_noalloc"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib

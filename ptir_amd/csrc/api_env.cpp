@@ -3,12 +3,41 @@
 
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 
 #include "internal.hpp"
 
 namespace {
 
+// Scratch of ptir_env_table_build_noalloc: FP64 cell weights, row sums and two totals for
+// tables of up to kNoAllocMaxCells cells and kNoAllocMaxRows rows (1.08 MB per device, held
+// until the process ends).  In host code `scratch` only names the symbol: its device
+// address, one per GPU, comes from hipGetSymbolAddress.
+constexpr int kNoAllocMaxCells = 256 * 512;
+constexpr int kNoAllocMaxRows = 4096;
+static __device__ __align__(256) double scratch[kNoAllocMaxCells + kNoAllocMaxRows + 2];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, resolved once per device.
+hipError_t noalloc_scratch(void** out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
 
 ptir_status env_fail(ptir_status st, const std::string& msg) {
     ptir::set_thread_error(msg);
@@ -47,6 +76,24 @@ ptir_status ptir_env_table_build(const float* environment, int32_t env_h, int32_
     e = ptir::env_table_build(environment, env_h, env_w, tab_h, tab_w, table, scratch, s);
     const hipError_t f = hipFreeAsync(scratch, s);
     if (e == hipSuccess) e = f;
+    if (e != hipSuccess) return env_fail(PTIR_ERR_HIP, std::string("environment table build: ") + hipGetErrorString(e));
+    return PTIR_OK;
+}
+
+ptir_status ptir_env_table_build_noalloc(const float* environment, int32_t env_h, int32_t env_w, int32_t tab_h,
+                                         int32_t tab_w, float* table, void* stream) {
+    ptir_status st = check_env_args(environment, env_h, env_w, tab_h, tab_w, table);
+    if (st != PTIR_OK) return st;
+    if ((int64_t)tab_h * tab_w > kNoAllocMaxCells || tab_h > kNoAllocMaxRows)
+        return env_fail(PTIR_ERR_INVALID_ARGUMENT,
+                        "ptir_env_table_build_noalloc takes tables of at most " + std::to_string(kNoAllocMaxCells) +
+                            " cells (" + std::to_string(tab_h) + " x " + std::to_string(tab_w) +
+                            " requested); use ptir_env_table_build");
+    void* ptr = nullptr;
+    hipError_t e = noalloc_scratch(&ptr);
+    if (e != hipSuccess) return env_fail(PTIR_ERR_HIP, std::string("resolving the static scratch: ") + hipGetErrorString(e));
+    // every scratch word is written by the build before it is read: nothing to clear
+    e = ptir::env_table_build(environment, env_h, env_w, tab_h, tab_w, table, ptr, static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return env_fail(PTIR_ERR_HIP, std::string("environment table build: ") + hipGetErrorString(e));
     return PTIR_OK;
 }
