@@ -275,6 +275,38 @@ ptir_status ptir_env_table_build(const float* environment, int32_t env_h, int32_
 ptir_status ptir_env_table_build_noalloc(const float* environment, int32_t env_h, int32_t env_w, int32_t tab_h,
                                          int32_t tab_w, float* table, void* stream);
 
+/* ------------------------------------------------------------------------------------
+ * Render-side image work on the device (the trainer-side torch code of the reference's
+ * Tracer.render).  Errors are reported by ptir_last_error(NULL).
+ *
+ * Camera rays for S samples per pixel (threedgptir_tracer/tracer.py:646-706): for sample s,
+ * view b and pixel (y, x), origin = ray_ori[b,y,x] and direction =
+ * normalize(((px - 0.5 + jx - cx) / fx, (py - 0.5 + jy - cy) / fy, 1)) with (px, py) =
+ * pixel_coords[b,y,x] ([B,H,W,2]), intrinsics = device [4] (fx, fy, cx, cy) and the jitter
+ * (jx, jy) = jitter[s] ([S,2]) or jitter[s,y,x] ([S,H,W,2], jitter_per_pixel = 1).  Output
+ * [S*B,H,W,3], sample-major. */
+ptir_status ptir_camera_rays(const float* pixel_coords, const float* intrinsics, const float* jitter,
+                             int32_t jitter_per_pixel, const float* ray_ori, int32_t S, int32_t B, int32_t H,
+                             int32_t W, float* out_ori, float* out_dir, void* stream);
+
+/* Resolve of the spp-averaged tracer outputs into render outputs (tracer.py:505-526,
+ * threedgrut/model/filters.py:6-69): out_dist = dist / opacity (non-finite -> 0),
+ * out_normals = normals / max(|normals|, 1e-12), and image [B,H,W,12] =
+ * (pbr * opacity, direct * opacity, indirect * opacity, light) filtered by `iterations`
+ * passes of the 3x3 B-spline a-trous kernel with replicate borders (0 = no filter).  work is
+ * a second [B,H,W,12] buffer.  The backward takes the packed gradient of image in `image`
+ * (overwritten) and writes every input gradient. */
+typedef struct ptir_resolve {
+    int32_t B, H, W, iterations;
+    const float *opacity, *dist, *pbr, *direct, *indirect, *light, *normals;
+    float *out_dist, *out_normals;
+    float *image, *work;
+} ptir_resolve;
+ptir_status ptir_resolve_forward(const ptir_resolve* r, void* stream);
+ptir_status ptir_resolve_backward(const ptir_resolve* r, const float* grad_dist, const float* grad_normals,
+                                  float* d_opacity, float* d_dist, float* d_pbr, float* d_direct, float* d_indirect,
+                                  float* d_light, float* d_normals, void* stream);
+
 /* Human-readable text of the last error on this handle (or, when tracer is NULL, of the
  * last failed call without a handle on this thread: ptir_create, the table builders).  Valid until the next call on the handle. */
 const char* ptir_last_error(const ptir_tracer* tracer);

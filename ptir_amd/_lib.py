@@ -26,7 +26,8 @@ EXPORTED = ["ptir_create", "ptir_destroy", "ptir_build_bvh", "ptir_trace", "ptir
             "ptir_last_error", "ptir_version", "ptir_cache_create", "ptir_cache_destroy", "ptir_cache_num_weights",
             "ptir_cache_table_floats", "ptir_cache_read_params", "ptir_cache_write_params", "ptir_cache_read_grads",
             "ptir_cache_query", "ptir_cache_backprop", "ptir_cache_step", "ptir_cache_train", "ptir_cache_last_error",
-            "ptir_env_table_scratch_bytes", "ptir_env_table_build", "ptir_env_table_build_noalloc"]
+            "ptir_env_table_scratch_bytes", "ptir_env_table_build", "ptir_env_table_build_noalloc",
+            "ptir_camera_rays", "ptir_resolve_forward", "ptir_resolve_backward"]
 
 
 class Options(C.Structure):
@@ -45,6 +46,12 @@ class Scene(C.Structure):
                 ("tab_kind", C.c_int32)]
 
 ENV_TABLE_ALIAS, ENV_TABLE_CDF = 0, 1
+
+
+class Resolve(C.Structure):
+    _fields_ = [("B", C.c_int32), ("H", C.c_int32), ("W", C.c_int32), ("iterations", C.c_int32)] + [
+        (k, C.c_void_p) for k in ("opacity", "dist", "pbr", "direct", "indirect", "light", "normals", "out_dist",
+                                  "out_normals", "image", "work")]
 
 
 class Launch(C.Structure):
@@ -127,10 +134,15 @@ def lib() -> C.CDLL:
         L.ptir_env_table_scratch_bytes.restype = u64
         L.ptir_env_table_build.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp, vp]
         L.ptir_env_table_build_noalloc.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp, vp]
+        i32 = C.c_int32
+        L.ptir_camera_rays.argtypes = [vp, vp, vp, i32, vp, i32, i32, i32, i32, vp, vp, vp]
+        L.ptir_resolve_forward.argtypes = [C.POINTER(Resolve), vp]
+        L.ptir_resolve_backward.argtypes = [C.POINTER(Resolve)] + [vp] * 10
         for name in ("ptir_create", "ptir_destroy", "ptir_build_bvh", "ptir_trace", "ptir_trace_bwd",
                      "ptir_cache_create", "ptir_cache_destroy", "ptir_cache_read_params", "ptir_cache_write_params",
                      "ptir_cache_read_grads", "ptir_cache_query", "ptir_cache_backprop", "ptir_cache_step", "ptir_cache_train",
-                     "ptir_env_table_build", "ptir_env_table_build_noalloc"):
+                     "ptir_env_table_build", "ptir_env_table_build_noalloc", "ptir_camera_rays", "ptir_resolve_forward",
+                     "ptir_resolve_backward"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib

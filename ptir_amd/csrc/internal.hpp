@@ -60,6 +60,22 @@ uint64_t env_table_scratch_bytes(int h, int w);
 hipError_t env_table_build(const float* env, int H, int W, int h, int w, float* table, void* scratch, hipStream_t s);
 void set_thread_error(const std::string& msg);
 
+// Render-side image work (render.hip).
+hipError_t camera_rays(const float* pix, const float* K, const float* jit, int jit_per_pixel, const float* ori, int S,
+                       int B, int H, int W, float* out_o, float* out_d, hipStream_t s);
+struct ResolveIO {
+    int B, H, W, iterations;                                   // iterations of the a-trous filter (0 = none)
+    const float *op, *dist, *pbr, *dir, *ind, *light, *nrm;    // [B,H,W,1|3] averaged tracer outputs
+    float *out_dist, *out_nrm;                                 // [B,H,W,1], [B,H,W,3]
+    float *img, *tmp;                                          // [B,H,W,12] work images
+};
+struct ResolveGrads {
+    const float *g_dist, *g_nrm;                               // incoming gradients of out_dist, out_nrm
+    float *d_op, *d_dist, *d_pbr, *d_dir, *d_ind, *d_light, *d_nrm;
+};
+hipError_t resolve_forward(const ResolveIO& io, hipStream_t s);
+hipError_t resolve_backward(const ResolveIO& io, const ResolveGrads& g, hipStream_t s);
+
 hipError_t launch_trace_fwd(const DeviceScene& S, const FwdOut& out, int kbuf, bool short_stack, hipStream_t s);
 hipError_t launch_trace_bwd(const DeviceScene& S, const BwdIO& io, int kbuf, bool short_stack, hipStream_t s);
 

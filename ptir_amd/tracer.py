@@ -16,14 +16,13 @@ from __future__ import annotations
 
 import ctypes as C
 import logging
-import math
 from dataclasses import dataclass
 from enum import IntEnum
 
 import torch
 
 from . import _lib
-from .filters import Filter, post_processing
+from . import envmap, render
 
 logger = logging.getLogger(__name__)
 
@@ -429,230 +428,184 @@ class Tracer:
         INDIRECT = 1
         DEFAULT = NONE
 
-    _MULTISPP_PATTERNS = {
-        1: ((0.5000, 0.5000),),
-        2: ((0.2500, 0.2500), (0.7500, 0.7500)),
-        4: ((0.3750, 0.1250), (0.8750, 0.3750), (0.6250, 0.8750), (0.1250, 0.6250)),
-        8: ((0.5625, 0.6875), (0.4375, 0.3125), (0.8125, 0.4375), (0.3125, 0.8125), (0.1875, 0.1875),
-            (0.0625, 0.5625), (0.6875, 0.0625), (0.9375, 0.9375)),
-        16: ((0.5625, 0.4375), (0.4375, 0.6875), (0.3125, 0.3750), (0.7500, 0.5625), (0.1875, 0.6250),
-             (0.6250, 0.1875), (0.1875, 0.3125), (0.6875, 0.8125), (0.3750, 0.1250), (0.5000, 0.9375),
-             (0.2500, 0.8750), (0.1250, 0.2500), (0.0000, 0.5000), (0.9375, 0.7500), (0.8750, 0.0625),
-             (0.0625, 0.0000)),
-        32: tuple(((x + 0.5) / 8.0, (y + 0.5) / 4.0) for y in range(4) for x in range(8)),
-    }
-
     def __init__(self, conf):
+        """Tracer(conf) of the reference (tracer.py:428-469): conf.render.* configures the
+        native tracer; conf.loss.* decides which outputs a training render must produce."""
         self.device = "cuda"
         self.conf = conf
         r = _conf_get(conf, "render")
         self.render_conf = r
-        self.num_update_bvh = 0
-        self._warned_spp_fallback = False
-        self.pred_pbr_filter = Filter(_conf_get(r, "filter_type", "none"))
+        self.loss_conf = _conf_get(conf, "loss")
+        self.filter_iterations = render.filter_iterations(_conf_get(r, "filter_type", "none"))
         self.visualize_lights = bool(_conf_get(r, "visualize_lights", False))
-        model = _conf_get(conf, "model")
-        enable_metallic = _conf_get(r, "enable_metallic", None)
-        if enable_metallic is None:
-            enable_metallic = _conf_get(model, "optimize_material_metallic", False)
+        self.density_clamping = bool(_conf_get(r, "particle_kernel_density_clamping", True))
+        self.max_bvh_updates = int(_conf_get(r, "max_consecutive_bvh_update", 15))
+        metallic = _conf_get(r, "enable_metallic", None)
+        if metallic is None:
+            metallic = _conf_get(_conf_get(conf, "model"), "optimize_material_metallic", False)
         self.tracer_wrapper = HipTracer(
-            "", "", _conf_get(r, "pipeline_type", "reference"),
-            _conf_get(r, "backward_pipeline_type", "referenceBwd"), _conf_get(r, "primitive_type", "instances"),
-            _conf_get(r, "particle_kernel_degree", 4), _conf_get(r, "particle_kernel_min_response", 0.0113),
-            _conf_get(r, "particle_kernel_density_clamping", True), _conf_get(r, "particle_radiance_sph_degree", 3),
-            _conf_get(r, "enable_normals", True), _conf_get(r, "enable_hitcounts", True),
-            _conf_get(r, "enable_mis", True), bool(enable_metallic), self.visualize_lights,
-            kbuffer=int(_conf_get(r, "kbuffer_depth", 16)))
+            "", "", _conf_get(r, "pipeline_type", "reference"), _conf_get(r, "backward_pipeline_type", "referenceBwd"),
+            _conf_get(r, "primitive_type", "instances"), _conf_get(r, "particle_kernel_degree", 4),
+            _conf_get(r, "particle_kernel_min_response", 0.0113), self.density_clamping,
+            _conf_get(r, "particle_radiance_sph_degree", 3), _conf_get(r, "enable_normals", True),
+            _conf_get(r, "enable_hitcounts", True), _conf_get(r, "enable_mis", True), bool(metallic),
+            self.visualize_lights, kbuffer=int(_conf_get(r, "kbuffer_depth", 16)),
+            env_sampling=str(_conf_get(r, "env_sampling", "alias")))
+        self.updates_since_rebuild = 0
+        self._built_geometry = None
+        self._spp_fallback_warned = False
         self.timings = {}
 
-    # -------------------------------------------------------------- helpers (tracer.py:505-714)
-    def _postprocess_pbr_outputs(self, outputs):
-        pred_opacity = outputs.get("pred_opacity")
-        keys = ("pred_pbr", "pred_direct", "pred_indirect")
-        if pred_opacity is not None:
-            for key in keys:
-                if outputs.get(key) is not None:
-                    outputs[key] = outputs[key] * pred_opacity
-        for key in (*keys, "pred_light"):
-            if outputs.get(key) is not None:
-                outputs[key] = self.pred_pbr_filter(outputs[key])
-
+    # -------------------------------------------------------------- acceleration structure
     @staticmethod
-    def _normalize_depth_output(outputs):
-        d, o = outputs.get("pred_dist"), outputs.get("pred_opacity")
-        if d is not None and o is not None:
-            outputs["pred_dist"] = torch.nan_to_num(d / o, 0.0, 0.0)
+    def _geometry_key(gaussians):
+        """Identity of the geometry tensors: storage, shape and in-place version counter.
+        An optimiser step bumps the version of every parameter it updates."""
+        key = []
+        for name in ("positions", "rotation", "scale", "density"):
+            t = getattr(gaussians, name)
+            key.append((t.data_ptr(), tuple(t.shape), t._version))
+        return tuple(key)
 
-    def _get_spp(self, train: bool) -> int:
-        key = "inversion_spp" if train else "render_spp"
-        return max(1, int(_conf_get(self.render_conf, key, 1)))
-
-    def _get_spp_chunk(self, spp: int) -> int:
-        return min(spp, max(1, int(_conf_get(self.render_conf, "spp_chunk", spp))))
-
-    @staticmethod
-    def _intrinsics(gpu_batch, dtype, device):
-        intr = getattr(gpu_batch, "intrinsics", None)
-        if intr is not None:
-            t = torch.as_tensor(intr, dtype=dtype, device=device).flatten()
-            if t.numel() == 4:
-                return tuple(t)
-            if t.numel() == 9:
-                K = t.reshape(3, 3)
-                return K[0, 0], K[1, 1], K[0, 2], K[1, 2]
-            return None
-        params = getattr(gpu_batch, "intrinsics_OpenCVPinholeCameraModelParameters", None)
-        if params is None:
-            return None
-
-        def get(name):
-            return params.get(name) if isinstance(params, dict) else getattr(params, name, None)
-
-        for name in ("radial_coeffs", "tangential_coeffs", "thin_prism_coeffs"):
-            c = get(name)
-            if c is not None:
-                c = torch.as_tensor(c, dtype=dtype, device=device).flatten()
-                if c.numel() > 0 and bool(torch.any(torch.abs(c) > 1e-8)):
-                    return None
-        focal, principal = get("focal_length"), get("principal_point")
-        if focal is None or principal is None:
-            return None
-        focal = torch.as_tensor(focal, dtype=dtype, device=device).flatten()
-        principal = torch.as_tensor(principal, dtype=dtype, device=device).flatten()
-        fx = focal[0]
-        fy = focal[1] if focal.numel() > 1 else focal[0]
-        return fx, fy, principal[0], principal[1]
-
-    def _make_spp_jitter(self, spp, h, w, device, dtype, frame_id):
-        if spp in self._MULTISPP_PATTERNS:
-            return torch.tensor(self._MULTISPP_PATTERNS[spp], dtype=dtype, device=device).view(spp, 1, 1, 2)
-        grid_w = int(math.ceil(math.sqrt(spp)))
-        grid_h = int(math.ceil(spp / grid_w))
-        idx = torch.arange(spp, dtype=dtype, device=device)
-        jitter = torch.stack(((torch.remainder(idx, grid_w) + 0.5) / grid_w,
-                              (torch.div(idx, grid_w, rounding_mode="floor") + 0.5) / grid_h), dim=-1)
-        g = torch.Generator(device=device)
-        g.manual_seed((int(frame_id) + 1) * 1315423911 + spp * 2654435761)
-        shift = torch.rand((1, 2), dtype=dtype, device=device, generator=g)
-        return torch.remainder(jitter + shift, 1.0).view(spp, 1, 1, 2)
-
-    def _expand_rays_for_spp(self, gpu_batch, spp, frame_id, jitter=None):
-        rays_ori = gpu_batch.rays_ori.contiguous()
-        rays_dir = gpu_batch.rays_dir.contiguous()
-        if spp == 1 and jitter is None:
-            return rays_ori, rays_dir, 1
-        pixel_coords = getattr(gpu_batch, "pixel_coords", None)
-        intr = self._intrinsics(gpu_batch, rays_dir.dtype, rays_dir.device)
-        if intr is None or pixel_coords is None:
-            return rays_ori, rays_dir, 1
-        base, h, w, _ = rays_dir.shape
-        fx, fy, cx, cy = intr
-        po = pixel_coords.to(dtype=rays_dir.dtype, device=rays_dir.device) - 0.5
-        if jitter is None:
-            jitter = self._make_spp_jitter(spp, h, w, rays_dir.device, rays_dir.dtype, frame_id)
-        po = po.unsqueeze(0)
-        jitter = jitter.view(spp, 1, 1, 1, 2) if jitter.shape[1:3] == (1, 1) else jitter.view(spp, 1, h, w, 2)
-        dirs = torch.stack(((po[..., 0] + jitter[..., 0] - cx) / fx, (po[..., 1] + jitter[..., 1] - cy) / fy,
-                            torch.ones((spp, base, h, w), dtype=rays_dir.dtype, device=rays_dir.device)), dim=-1)
-        dirs = torch.nn.functional.normalize(dirs, dim=-1).reshape(spp * base, h, w, 3).contiguous()
-        origins = rays_ori.unsqueeze(0).expand(spp, *rays_ori.shape).reshape(spp * base, h, w, 3).contiguous()
-        return origins, dirs, spp
-
-    @staticmethod
-    def _average_spp_output(value, spp, base_batch):
-        if spp == 1:
-            return value
-        return value.reshape(spp, base_batch, *value.shape[1:]).mean(dim=0)
-
-    # -------------------------------------------------------------- build_acc (tracer.py:716-738)
     def build_acc(self, gaussians, rebuild=True):
-        r = self.render_conf
-        max_upd = int(_conf_get(r, "max_consecutive_bvh_update", 15))
-        clamping = bool(_conf_get(r, "particle_kernel_density_clamping", True))
-        allow_update = max_upd > 1 and not clamping
-        rebuild_bvh = rebuild or clamping or self.num_update_bvh >= max_upd
+        """build_acc of the reference (tracer.py:716-738): a full rebuild when asked, when
+        density clamping is on, or after max_consecutive_bvh_update refits, else a refit.
+        When the geometry tensors are exactly those of the last build (PTIR's inversion stage
+        freezes geometry, configs/base_ptir.yaml:13), nothing is rebuilt: the hierarchy
+        already describes them."""
+        key = self._geometry_key(gaussians)
+        if key == self._built_geometry:
+            return
+        full = bool(rebuild) or self.density_clamping or self.updates_since_rebuild >= self.max_bvh_updates
+        allow_update = self.max_bvh_updates > 1 and not self.density_clamping
+        if self._built_geometry is None:
+            full = True
         self.tracer_wrapper.build_bvh(
             gaussians.positions.view(-1, 3).contiguous(),
             gaussians.rotation_activation(gaussians.rotation).view(-1, 4).contiguous(),
             gaussians.scale_activation(gaussians.scale).view(-1, 3).contiguous(),
-            gaussians.density_activation(gaussians.density).view(-1, 1).contiguous(),
-            rebuild_bvh, allow_update)
-        self.num_update_bvh = 0 if rebuild_bvh else self.num_update_bvh + 1
+            gaussians.density_activation(gaussians.density).view(-1, 1).contiguous(), full, allow_update)
+        self.updates_since_rebuild = 0 if full else self.updates_since_rebuild + 1
+        self._built_geometry = key
 
-    # -------------------------------------------------------------- render (tracer.py:740-975)
-    def render(self, gaussians, gpu_batch, train=False, frame_id=0, sh_indirect: bool = False):
-        base_batch = gpu_batch.rays_dir.shape[0]
-        spp = self._get_spp(train)
-        if spp > 1 and (getattr(gpu_batch, "pixel_coords", None) is None or
-                        self._intrinsics(gpu_batch, gpu_batch.rays_dir.dtype, gpu_batch.rays_dir.device) is None):
-            if not self._warned_spp_fallback:
-                logger.warning("PTIR SPP fallback to spp=1: batch has no supported pinhole intrinsics/pixel_coords")
-                self._warned_spp_fallback = True
-            spp = 1
-        spp_chunk = self._get_spp_chunk(spp)
-        h, w = gpu_batch.rays_dir.shape[1:3]
-        spp_jitter = None
+    # -------------------------------------------------------------- render
+    def _wanted_outputs(self, train: bool):
+        """(outputs returned, outputs accumulated) of a render; None = all.  Only a training
+        render with render.hide_intermediate_outputs drops outputs (tracer.py:470-503)."""
+        if not train or not bool(_conf_get(self.render_conf, "hide_intermediate_outputs", False)):
+            return None, None
+        keep = {"pred_pbr", "pred_material"}
+        lc = self.loss_conf
+        if bool(_conf_get(lc, "use_light_consistency", False)) or float(_conf_get(lc, "lambda_light", 0.0)) > 0.0:
+            keep.add("pred_light")
+        if bool(_conf_get(lc, "use_edge_aware_smoothness", False)):
+            keep.update(_conf_get(lc, "edge_aware_smoothness_outputs", []) or [])
+        acc = set(keep)
+        if "pred_pbr" in acc:
+            acc.add("pred_opacity")
+        if acc & {"pred_direct", "pred_indirect"}:
+            acc.update(("pbr_components", "pred_opacity"))
+        return keep, acc
+
+    def _spp_plan(self, batch, train: bool, frame_id: int) -> render.SppPlan:
+        spp = max(1, int(_conf_get(self.render_conf, "inversion_spp" if train else "render_spp", 1)))
+        dev = batch.rays_dir.device
+        intr = pinhole = None
         if spp > 1:
-            spp_jitter = self._make_spp_jitter(spp, h, w, gpu_batch.rays_dir.device, gpu_batch.rays_dir.dtype,
-                                               frame_id)
-        acc = {}
-        mog_visibility = None
-        total_spp = 0
-        for spp_start in range(0, spp, spp_chunk):
-            chunk_spp = min(spp_chunk, spp - spp_start)
-            chunk_jitter = None if spp_jitter is None else spp_jitter[spp_start:spp_start + chunk_spp]
-            rays_ori, rays_dir, chunk_spp = self._expand_rays_for_spp(gpu_batch, chunk_spp, frame_id + spp_start,
-                                                                      jitter=chunk_jitter)
-            environment = gaussians.get_environment()
-            if environment is None:
-                environment = torch.empty((0, 0, 4), dtype=torch.float32, device=rays_dir.device)
-            table = getattr(gaussians, "environment_alias_table", None)
-            if table is None:
-                env_table = torch.empty(0, dtype=torch.float32, device=rays_dir.device)
+            pinhole = getattr(batch, "pixel_coords", None) is not None
+            intr = render.pinhole_intrinsics(batch, dev) if pinhole else None
+            if intr is None:
+                if not self._spp_fallback_warned:
+                    logger.warning("spp %d needs pixel_coords and distortion-free pinhole intrinsics on the batch; "
+                                   "rendering 1 sample per pixel", spp)
+                    self._spp_fallback_warned = True
+                spp = 1
+        chunk = min(spp, max(1, int(_conf_get(self.render_conf, "spp_chunk", spp))))
+        pos = render.sample_positions(spp, frame_id, dev, batch.rays_dir.dtype) if spp > 1 else None
+        return render.SppPlan(spp, chunk, pos, intr)
+
+    @staticmethod
+    def _environment(gaussians, tracer_wrapper, device):
+        env = gaussians.get_environment()
+        if env is None:
+            return torch.empty((0, 0, 4), dtype=torch.float32, device=device), torch.empty(0, device=device)
+        if tracer_wrapper.tab_kind == _lib.ENV_TABLE_CDF:
+            return env, envmap.build_cdf_table(env.detach().contiguous())
+        t = getattr(gaussians, "environment_alias_table", None)
+        if t is None:
+            return env, torch.empty(0, dtype=torch.float32, device=device)
+        planes = [t.prob, t.alias.to(torch.float32), t.pdf]
+        table = torch.stack([p.reshape(t.height, t.width).float() for p in planes]).to(device).contiguous()
+        return env, table
+
+    def render(self, gaussians, gpu_batch, train=False, frame_id=0, sh_indirect: bool = False):
+        """Tracer.render of the reference (tracer.py:740-975): samples per pixel in chunks,
+        each chunk one trace through the autograd function, the model's background on the
+        radiance, the spp average, then the resolve of PBR outputs on the device."""
+        keep, acc_keys = self._wanted_outputs(train)
+        base = gpu_batch.rays_dir.shape[0]
+        dev = gpu_batch.rays_dir.device
+        plan = self._spp_plan(gpu_batch, train, frame_id)
+        env, table = self._environment(gaussians, self.tracer_wrapper, dev)
+        lights = LightBuffers.from_batch(gpu_batch, device=dev)
+        to_world = gpu_batch.T_to_world.contiguous()
+        opts = int(Tracer.RenderOpts.INDIRECT if sh_indirect else Tracer.RenderOpts.DEFAULT)
+        names = ("pred_rgb", "pred_opacity", "pred_dist", "pred_depth_second_moment", "pred_depth_distortion",
+                 "pred_normals", "pred_shadingnormal", "pred_material", "hits_count", "pred_pbr", "pred_light",
+                 "pbr_components")
+        sums: dict = {}
+        visibility = None
+        for start, count in plan.chunks():
+            if plan.positions is None:
+                ori, dirs = gpu_batch.rays_ori.contiguous(), gpu_batch.rays_dir.contiguous()
             else:
-                env_table = torch.concat([
-                    table.prob.reshape(1, table.height, table.width),
-                    table.alias.reshape(1, table.height, table.width).to(dtype=torch.float32),
-                    table.pdf.reshape(1, table.height, table.width)], dim=0).to(device=rays_dir.device).contiguous()
-            light_buffers = LightBuffers.from_batch(gpu_batch, device=rays_dir.device)
-            outs = Tracer._Autograd.apply(
-                self.tracer_wrapper, frame_id + spp_start, gpu_batch.T_to_world.contiguous(), rays_ori, rays_dir,
-                gaussians.positions.contiguous(), gaussians.get_rotation().contiguous(),
-                gaussians.get_scale().contiguous(), gaussians.get_density().contiguous(),
-                gaussians.get_features().contiguous(), gaussians.get_shading_normal().contiguous(),
-                gaussians.get_material_albedo().contiguous(), gaussians.get_material_roughness().contiguous(),
-                gaussians.get_material_metallic().contiguous(), environment, env_table, light_buffers,
-                int(Tracer.RenderOpts.INDIRECT if sh_indirect else Tracer.RenderOpts.DEFAULT),
-                gaussians.n_active_features, _conf_get(self.render_conf, "min_transmittance", 1e-3),
-                _conf_get(self.render_conf, "max_bounces", 3))
-            (c_rgb, c_opacity, c_dist, c_dist2, c_distortion, c_normals, c_snormal, c_material, c_hits, c_vis, c_pbr,
-             c_light, c_comp) = outs
-            background = getattr(gaussians, "background", None)
-            if callable(background):
-                c_rgb, c_opacity = background(gpu_batch.T_to_world.contiguous(), rays_dir, c_rgb, c_opacity, train)
-            for name, value in (("pred_rgb", c_rgb), ("pred_opacity", c_opacity), ("pred_dist", c_dist),
-                                ("pred_depth_second_moment", c_dist2), ("pred_depth_distortion", c_distortion),
-                                ("pred_normals", c_normals), ("pred_shadingnormal", c_snormal),
-                                ("pred_material", c_material), ("hits_count", c_hits), ("pred_pbr", c_pbr),
-                                ("pred_light", c_light), ("pbr_components", c_comp.detach())):
-                avg = self._average_spp_output(value, chunk_spp, base_batch) * chunk_spp
-                acc[name] = avg if name not in acc else acc[name] + avg
-            mog_visibility = c_vis if mog_visibility is None else torch.maximum(mog_visibility, c_vis)
-            total_spp += chunk_spp
-        acc = {k: v / total_spp for k, v in acc.items()}
-        comp = acc.pop("pbr_components", None)
-        if comp is not None:
-            acc["pred_direct"] = comp[..., 0, :]
-            acc["pred_indirect"] = comp[..., 1, :]
-        self._normalize_depth_output(acc)
-        self._postprocess_pbr_outputs(acc)
-        keys = ("pred_rgb", "pred_opacity", "pred_dist", "pred_depth_second_moment", "pred_depth_distortion",
-                "pred_normals", "pred_shadingnormal", "pred_material", "pred_pbr", "pred_light", "pred_direct",
-                "pred_indirect", "hits_count")
-        out = {k: acc.get(k) for k in keys}
-        if out["pred_normals"] is not None:
-            out["pred_normals"] = torch.nn.functional.normalize(out["pred_normals"], dim=3)
-        out = {k: v for k, v in out.items() if v is not None}
+                ori, dirs = render.camera_rays(gpu_batch, plan.positions[start:start + count], plan.intrinsics)
+            out = Tracer._Autograd.apply(
+                self.tracer_wrapper, frame_id + start, to_world, ori, dirs, gaussians.positions.contiguous(),
+                gaussians.get_rotation().contiguous(), gaussians.get_scale().contiguous(),
+                gaussians.get_density().contiguous(), gaussians.get_features().contiguous(),
+                gaussians.get_shading_normal().contiguous(), gaussians.get_material_albedo().contiguous(),
+                gaussians.get_material_roughness().contiguous(), gaussians.get_material_metallic().contiguous(),
+                env, table, lights, opts, gaussians.n_active_features,
+                _conf_get(self.render_conf, "min_transmittance", 1e-3), _conf_get(self.render_conf, "max_bounces", 3))
+            rgb, opacity = gaussians.background(to_world, dirs, out[0], out[1], train)
+            values = (rgb, opacity, *out[2:9], out[10], out[11], out[12].detach())
+            for name, v in zip(names, values):
+                if acc_keys is not None and name not in acc_keys:
+                    continue
+                # samples of a chunk are the leading factor of the batch: sum them
+                total = v.reshape(count, base, *v.shape[1:]).sum(dim=0)
+                sums[name] = total if name not in sums else sums[name] + total
+            visibility = out[9] if visibility is None else torch.maximum(visibility, out[9])
+        mean = {k: v / plan.spp for k, v in sums.items()}
+        res = self._resolve(mean)
+        out = {k: res[k] for k in ("pred_rgb", "pred_opacity", "pred_dist", "pred_depth_second_moment",
+                                   "pred_depth_distortion", "pred_normals", "pred_shadingnormal", "pred_material",
+                                   "pred_pbr", "pred_light", "pred_direct", "pred_indirect", "hits_count")
+               if res.get(k) is not None and (keep is None or k in keep)}
         out["frame_time_ms"] = 0.0
-        out["mog_visibility"] = mog_visibility
-        return post_processing(out, gpu_batch, self.visualize_lights)
+        out["mog_visibility"] = visibility
+        return render.apply_gt_mask(out, gpu_batch)
+
+    def _resolve(self, mean: dict) -> dict:
+        """Depth / opacity, opacity-weighted and filtered PBR terms, unit normals."""
+        res = dict(mean)
+        comp = res.pop("pbr_components", None)
+        op = mean.get("pred_opacity")
+        if op is None:
+            if comp is not None:
+                res["pred_direct"], res["pred_indirect"] = comp[..., 0, :], comp[..., 1, :]
+            return res
+        z3 = torch.zeros(op.shape[:3] + (3,), dtype=torch.float32, device=op.device)
+        pick = lambda k: mean[k] if mean.get(k) is not None else z3  # noqa: E731
+        direct = comp[..., 0, :] if comp is not None else z3
+        indirect = comp[..., 1, :] if comp is not None else z3
+        dist = mean.get("pred_dist", torch.zeros_like(op))
+        d, pbr, dr, ind, light, nrm = render.Resolve.apply(self.filter_iterations, op, dist, pick("pred_pbr"), direct,
+                                                           indirect, pick("pred_light"), pick("pred_normals"))
+        for k, v in (("pred_dist", d), ("pred_pbr", pbr), ("pred_light", light), ("pred_normals", nrm)):
+            if mean.get(k) is not None:
+                res[k] = v
+        if comp is not None:
+            res["pred_direct"], res["pred_indirect"] = dr, ind
+        return res

@@ -9,7 +9,6 @@ import pytest
 import torch
 
 from ptir_amd import _lib, envmap, scenes
-from ptir_amd.filters import Filter, apply_gt_mask_to_tensor, post_processing
 from ptir_amd.tracer import HipTracer, LightBuffers, Tracer
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -99,57 +98,6 @@ def test_light_buffers_empty_shapes():
     shapes = [tuple(x.shape) for x in lb.tensors()]
     assert shapes == [(0, 9), (5, 0), (0, 3), (0, 3), (0, 8), (3, 0)]
     assert lb.mesh_triangles.dtype == torch.int32
-
-
-def test_atrous_filter_matches_direct_convolution():
-    x = torch.rand(2, 9, 7, 3, generator=torch.Generator().manual_seed(0))
-    y = Filter("atrous")(x)
-    k = torch.tensor([[1, 2, 1], [2, 4, 2], [1, 2, 1]], dtype=torch.float32) / 16
-    xp = torch.nn.functional.pad(x.permute(0, 3, 1, 2), (1, 1, 1, 1), mode="replicate").permute(0, 2, 3, 1)
-    ref = torch.zeros_like(x)
-    for dy in range(3):
-        for dx in range(3):
-            ref += k[dy, dx] * xp[:, dy:dy + 9, dx:dx + 7]
-    torch.testing.assert_close(y, ref, rtol=1e-6, atol=1e-6)
-    assert torch.equal(Filter("none")(x), x)
-    with pytest.raises(ValueError):
-        Filter("gaussian")
-
-
-def test_gt_mask_only_touches_material_light_and_shading_normal():
-    class B:
-        mask = torch.zeros(1, 4, 4, 1)
-
-    outs = {k: torch.ones(1, 4, 4, 3) for k in ("pred_pbr", "pred_light", "pred_material", "pred_shadingnormal")}
-    res = post_processing(outs, B())
-    assert torch.equal(res["pred_pbr"], outs["pred_pbr"])
-    for k in ("pred_light", "pred_material", "pred_shadingnormal"):
-        assert float(res[k].abs().sum()) == 0
-    assert res["_pbr_gt_mask_applied"]
-    assert apply_gt_mask_to_tensor(torch.ones(1, 4, 4, 3), torch.ones(4, 4)).shape == (1, 4, 4, 3)
-
-
-def test_spp_expansion_follows_the_reference_patterns():
-    class Batch:
-        pass
-
-    h, w = 6, 5
-    b = Batch()
-    b.rays_dir = torch.zeros(1, h, w, 3)
-    b.rays_ori = torch.zeros(1, h, w, 3)
-    ys, xs = torch.meshgrid(torch.arange(h) + 0.5, torch.arange(w) + 0.5, indexing="ij")
-    b.pixel_coords = torch.stack([xs, ys], -1).unsqueeze(0)
-    b.intrinsics = [10.0, 10.0, w / 2, h / 2]
-    tr = Tracer.__new__(Tracer)
-    o, d, spp = tr._expand_rays_for_spp(b, 4, 0)
-    assert spp == 4 and d.shape == (4, h, w, 3)
-    torch.testing.assert_close(d.norm(dim=-1), torch.ones(4, h, w))
-    # sample 0 of the 4-spp pattern sits at (0.375, 0.125) inside the pixel
-    px = torch.tensor([2 + 0.375 - w / 2, 3 + 0.125 - h / 2, 10.0]) / 10.0
-    torch.testing.assert_close(d[0, 3, 2], px / px.norm())
-    # the scenes module uses the same pattern table
-    for k, v in scenes.MULTISPP_PATTERNS.items():
-        assert Tracer._MULTISPP_PATTERNS[k] == v
 
 
 def test_env_table_packing_layout():

@@ -388,3 +388,31 @@ def test_autograd_function_gradients(small, device):
     got = torch.cat([alb.grad, rgh.grad, met.grad], 1)
     assert rel_l2(got.cpu().numpy(), ref[1].cpu().numpy()) < 1e-5
     assert rel_l2(env.grad.cpu().numpy(), ref[4].cpu().numpy()) < 1e-5
+
+
+def test_refit_after_moving_gaussians_equals_rebuild(device):
+    """build_bvh(rebuild=False) keeps the topology and refits the boxes
+    (OptixTracer::buildBVH update path, optixTracer.cpp:633-907).  The candidate rule is exact
+    and hits are ordered by distance, so a refitted hierarchy gives the trace of a rebuilt
+    one, also after the Gaussians moved."""
+    sc = scenes.small_scene(2, n=2048, res=32, spp=1)
+    tab = env_table(sc)
+    g = sc.to(device)
+    gen = torch.Generator().manual_seed(9)
+    moved = (g.positions + 0.05 * torch.randn(g.positions.shape, generator=gen).to(device)).contiguous()
+    dens = torch.cat([moved, g.density, g.rotation, g.scale, torch.zeros_like(g.density)], 1).contiguous()
+    args = (0, g.ray_to_world, g.rays_ori, g.rays_dir, dens, g.particle_material(), g.features, g.shading_normal,
+            g.env, tab.to(device))
+    refit = HipTracer(enable_metallic=True)
+    refit.build_bvh(g.positions, g.rotation, g.scale, g.density)
+    refit.build_bvh(moved, g.rotation, g.scale, g.density, rebuild=False)
+    rebuilt = HipTracer(enable_metallic=True)
+    rebuilt.build_bvh(moved, g.rotation, g.scale, g.density)
+    a = refit.trace(*args, max_bounces=2)
+    b = rebuilt.trace(*args, max_bounces=2)
+    torch.cuda.synchronize()
+    for x, y in zip(a, b):
+        assert torch.equal(x, y)
+    # refitting with another number of Gaussians is an error
+    with pytest.raises(Exception):
+        refit.build_bvh(moved[:10], g.rotation[:10], g.scale[:10], g.density[:10], rebuild=False)
