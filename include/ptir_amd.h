@@ -96,7 +96,30 @@ typedef struct ptir_launch {
      * throughput).  1 and 2 need max_bounces >= 3 and ptir_fwd_outputs.cache_records.
      * ptir_trace_bwd must be given the same value. */
     uint32_t cache_mode;
+    /* Ragged batch (no counterpart in the reference, which traces one [B,H,W] batch under one
+     * camera transform, optixTracer.cpp:962): num_images > 0 traces images of different
+     * sizes, e.g. several views at their own resolutions and samples per pixel, in one
+     * launch.  images is a DEVICE int32 [num_images, 4] table of rows (H, W, first tile,
+     * first path) filled by ptir_ragged_layout; rays and outputs are flat, [num_paths, C] in
+     * image order (pixel (y, x) of image i is path first_path_i + y W_i + x, and its
+     * launch index, i.e. its random stream, is (x, y, i)).  B, H, W are ignored; use world-
+     * space rays with an identity ray_to_world.  ptir_trace_bwd takes the same fields. */
+    int32_t num_images;
+    const int32_t* images;
+    int32_t num_tiles;
+    int32_t num_paths;
+    /* Russian roulette (no counterpart in the reference): 0 = off; else, after sampling the
+     * direction that leaves path vertex rr_start or later, the path continues with
+     * probability q = min(1, max throughput) and divides its throughput by q (one random
+     * draw, replayed by the backward, which takes 1/q as a constant). */
+    uint32_t rr_start;
 } ptir_launch;
+
+/* Host-side layout of a ragged batch of n images of sizes hw[2i] x hw[2i+1] (H, W):
+ * writes the [n, 4] table rows (H, W, first tile, first path) into table (host memory; the
+ * caller copies it to the device) and the totals.  The tracer covers each image with 16x8
+ * pixel tiles. */
+ptir_status ptir_ragged_layout(const int32_t* hw, int32_t n, int32_t* table, int32_t* num_tiles, int32_t* num_paths);
 
 /* Forward outputs, all [B,H,W,C] float32 unless noted (optixTracer.cpp:931-944).
  * The library writes every element: G-buffer outputs always, pbr/light/components zero
@@ -306,6 +329,29 @@ ptir_status ptir_resolve_forward(const ptir_resolve* r, void* stream);
 ptir_status ptir_resolve_backward(const ptir_resolve* r, const float* grad_dist, const float* grad_normals,
                                   float* d_opacity, float* d_dist, float* d_pbr, float* d_direct, float* d_indirect,
                                   float* d_light, float* d_normals, void* stream);
+
+/* ------------------------------------------------------------------------------------
+ * Adam step of the optimised tensors (the optimiser of the reference's PTIR inversion:
+ * torch.optim.Adam, or SelectiveAdam of threedgrut/optimizers/optimizers.cu:49-80 with
+ * visibility-masked rows and no bias correction).  Each group is a [rows, cols] float32
+ * tensor with its gradient and moment buffers (device pointers) and learning rate;
+ * visibility, if not NULL, is a [rows] array whose nonzero words mark the rows to update
+ * (the tracer's visibility output).  bias_correction = 1: torch.optim.Adam,
+ * p -= (lr / (1 - b1^t)) m / (sqrt(v) / sqrt(1 - b2^t) + eps) with t = *step + 1;
+ * 0: p -= lr m / (sqrt(v) + eps).  step is a device int32 incremented by the call, so the
+ * step can be captured in a graph and replayed.  Errors: ptir_last_error(NULL). */
+typedef struct ptir_adam_group {
+    float* param;
+    const float* grad;
+    float* exp_avg;
+    float* exp_avg_sq;
+    const uint32_t* visibility;
+    uint64_t rows;
+    uint32_t cols;
+    float lr;
+} ptir_adam_group;
+ptir_status ptir_adam_step(const ptir_adam_group* groups, int32_t num_groups, float beta1, float beta2, float eps,
+                           int32_t bias_correction, int32_t* step, void* stream);
 
 /* Human-readable text of the last error on this handle (or, when tracer is NULL, of the
  * last failed call without a handle on this thread: ptir_create, the table builders).  Valid until the next call on the handle. */

@@ -31,13 +31,14 @@ class OracleScene(C.Structure):
         ("sph_degree", C.c_int32), ("enable_normals", C.c_int32), ("enable_hitcounts", C.c_int32),
         ("enable_mis", C.c_int32), ("enable_metallic", C.c_int32), ("render_opts", C.c_int32),
         ("min_transmittance", C.c_float), ("max_bounces", C.c_uint32), ("frame", C.c_uint32), ("kbuf", C.c_int32),
-        ("tab_kind", C.c_int32),
+        ("tab_kind", C.c_int32), ("rr_start", C.c_uint32),
     ]
 
 
 class OracleRays(C.Structure):
     _fields_ = [("B", C.c_int32), ("H", C.c_int32), ("W", C.c_int32), ("ray_to_world", C.c_void_p),
-                ("ori", C.c_void_p), ("dir", C.c_void_p), ("path_stride", C.c_int64), ("path_offset", C.c_int64)]
+                ("ori", C.c_void_p), ("dir", C.c_void_p), ("path_stride", C.c_int64), ("path_offset", C.c_int64),
+                ("z_offset", C.c_int64)]
 
 
 _FWD_KEYS = ["rad", "dns", "hit", "hit2", "distortion", "nrm", "snrm", "mat", "hits", "pbr", "light", "comp",
@@ -121,7 +122,7 @@ class Inputs:
     def __init__(self, density, material, sh, snormal, env, table, *, sph_degree=3, kernel_degree=4.0,
                  min_response=0.0113, density_clamping=True, enable_normals=True, enable_hitcounts=True,
                  enable_mis=True, enable_metallic=False, render_opts=0, min_transmittance=1e-3, max_bounces=3,
-                 frame=0, kbuf=16, tab_kind=0):
+                 frame=0, kbuf=16, tab_kind=0, rr_start=0):
         self.density = _f32(density).reshape(-1, 12)
         self.material = _f32(material).reshape(-1, 5)
         self.sh = _f32(sh).reshape(self.density.shape[0], -1)
@@ -143,6 +144,7 @@ class Inputs:
         s.enable_mis, s.enable_metallic, s.render_opts = int(enable_mis), int(enable_metallic), int(render_opts)
         s.min_transmittance, s.max_bounces, s.frame, s.kbuf = min_transmittance, max_bounces, frame, kbuf
         s.tab_kind = int(tab_kind)
+        s.rr_start = int(rr_start)
         self.s = s
 
     @classmethod
@@ -155,7 +157,7 @@ class Inputs:
 
 
 class Rays:
-    def __init__(self, ori, dir, ray_to_world, path_stride=1, path_offset=0):
+    def __init__(self, ori, dir, ray_to_world, path_stride=1, path_offset=0, z_offset=0):
         self.ori = _f32(ori)
         self.dir = _f32(dir)
         self.T = _f32(ray_to_world).reshape(-1)[:12].copy()
@@ -164,6 +166,7 @@ class Rays:
         r.B, r.H, r.W = B, H, W
         r.ray_to_world, r.ori, r.dir = _ptr(self.T), _ptr(self.ori), _ptr(self.dir)
         r.path_stride, r.path_offset = path_stride, path_offset
+        r.z_offset = z_offset
         self.r = r
         self.shape = (B, H, W)
 

@@ -170,6 +170,7 @@ struct OracleScene {
     uint32_t frame;
     int32_t kbuf;            // hits per chunk (16 in the reference)
     int32_t tab_kind;        // 0: alias table (prob, alias, pdf); 1: CDF table (conditional, marginal, pdf)
+    uint32_t rr_start;       // Russian roulette from this path vertex on (0 = off); no reference counterpart
 };
 
 struct OracleRays {
@@ -179,6 +180,7 @@ struct OracleRays {
     const float* dir;           // [B,H,W,3]
     int64_t path_stride;        // evaluate every path_stride-th path (1 = all)
     int64_t path_offset;
+    int64_t z_offset;           // added to the image index of the launch index (x, y, z)
 };
 
 // Outputs in double, [B,H,W,C] like the reference's 13 outputs (src/optixTracer.cpp:931-944).
@@ -1285,10 +1287,24 @@ struct Tracer {
         return ray;
     }
 
+    // Russian roulette after the BRDF sample leaving vertex numBounces >= rr_start (an
+    // extension; the reference has none): survive with q = min(1, max throughput), then
+    // divide the throughput by q.  The backward replays the same draw.
+    bool roulette(Path<R>& p, Pcg32& rng) {
+        if (in.rr_start == 0u || p.numBounces < in.rr_start) return true;
+        const R q = std::min(R(1), vmax(p.thr));
+        const R u = R(rng.next_1d());
+        note(double(u), double(q));
+        if (!(u < q)) return false;
+        p.thr = p.thr * (R(1) / q);
+        return true;
+    }
+
     // referenceOptix.cu:27-71
     void forward(const OracleRays& r, int64_t pix, const OracleFwdOut& out) {
         const int64_t W = r.W, H = r.H;
-        const uint32_t x = uint32_t(pix % W), y = uint32_t((pix / W) % H), z = uint32_t(pix / (W * H));
+        const uint32_t x = uint32_t(pix % W), y = uint32_t((pix / W) % H),
+                       z = uint32_t(pix / (W * H) + r.z_offset);
         Pcg32 rng;
         rng.seed(Pcg32::launch_seed(x, y, z, in.frame), 0);
         Path<R> p;
@@ -1323,6 +1339,7 @@ struct Tracer {
             sample_next(p, rng);
             if (!is_zero(p.thr)) note(double(vmax(p.thr)), 1e-4);
             if (vmax(p.thr) < R(1e-4)) break;
+            if (!roulette(p, rng)) break;
             ray_intersect(p.cur.ray, p.cur, rng, true, false);
         }
         if (out.margin) out.margin[pix] = margin;
@@ -1335,7 +1352,8 @@ struct Tracer {
     // referenceBwdOptix.cu:27-74
     void backward(const OracleRays& r, int64_t pix, const OracleBwdIn& bi) {
         const int64_t W = r.W, H = r.H;
-        const uint32_t x = uint32_t(pix % W), y = uint32_t((pix / W) % H), z = uint32_t(pix / (W * H));
+        const uint32_t x = uint32_t(pix % W), y = uint32_t((pix / W) % H),
+                       z = uint32_t(pix / (W * H) + r.z_offset);
         Pcg32 rng;
         rng.seed(Pcg32::launch_seed(x, y, z, in.frame), 0);
         Path<R> p;
@@ -1362,6 +1380,7 @@ struct Tracer {
             p.numBounces = depth + 1u;
             sample_next_bwd(p, rng);
             if (vmax(p.thr) < R(1e-4)) break;
+            if (!roulette(p, rng)) break;
             ray_intersect(p.cur.ray, p.cur, rng, true, false);
         }
     }

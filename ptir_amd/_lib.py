@@ -27,7 +27,8 @@ EXPORTED = ["ptir_create", "ptir_destroy", "ptir_build_bvh", "ptir_trace", "ptir
             "ptir_cache_table_floats", "ptir_cache_read_params", "ptir_cache_write_params", "ptir_cache_read_grads",
             "ptir_cache_query", "ptir_cache_backprop", "ptir_cache_step", "ptir_cache_train", "ptir_cache_last_error",
             "ptir_env_table_scratch_bytes", "ptir_env_table_build", "ptir_env_table_build_noalloc",
-            "ptir_camera_rays", "ptir_resolve_forward", "ptir_resolve_backward"]
+            "ptir_camera_rays", "ptir_resolve_forward", "ptir_resolve_backward", "ptir_adam_step",
+            "ptir_ragged_layout"]
 
 
 class Options(C.Structure):
@@ -48,6 +49,11 @@ class Scene(C.Structure):
 ENV_TABLE_ALIAS, ENV_TABLE_CDF = 0, 1
 
 
+class AdamGroup(C.Structure):
+    _fields_ = [("param", C.c_void_p), ("grad", C.c_void_p), ("exp_avg", C.c_void_p), ("exp_avg_sq", C.c_void_p),
+                ("visibility", C.c_void_p), ("rows", C.c_uint64), ("cols", C.c_uint32), ("lr", C.c_float)]
+
+
 class Resolve(C.Structure):
     _fields_ = [("B", C.c_int32), ("H", C.c_int32), ("W", C.c_int32), ("iterations", C.c_int32)] + [
         (k, C.c_void_p) for k in ("opacity", "dist", "pbr", "direct", "indirect", "light", "normals", "out_dist",
@@ -58,7 +64,8 @@ class Launch(C.Structure):
     _fields_ = [("frame", C.c_uint32), ("ray_to_world", C.c_void_p), ("ray_ori", C.c_void_p), ("ray_dir", C.c_void_p),
                 ("B", C.c_int32), ("H", C.c_int32), ("W", C.c_int32), ("render_opts", C.c_uint32),
                 ("sph_degree", C.c_int32), ("min_transmittance", C.c_float), ("max_bounces", C.c_uint32),
-                ("cache_mode", C.c_uint32)]
+                ("cache_mode", C.c_uint32), ("num_images", C.c_int32), ("images", C.c_void_p),
+                ("num_tiles", C.c_int32), ("num_paths", C.c_int32), ("rr_start", C.c_uint32)]
 
 
 class FwdOutputs(C.Structure):
@@ -138,11 +145,13 @@ def lib() -> C.CDLL:
         L.ptir_camera_rays.argtypes = [vp, vp, vp, i32, vp, i32, i32, i32, i32, vp, vp, vp]
         L.ptir_resolve_forward.argtypes = [C.POINTER(Resolve), vp]
         L.ptir_resolve_backward.argtypes = [C.POINTER(Resolve)] + [vp] * 10
+        L.ptir_adam_step.argtypes = [C.POINTER(AdamGroup), i32, C.c_float, C.c_float, C.c_float, i32, vp, vp]
+        L.ptir_ragged_layout.argtypes = [vp, i32, vp, vp, vp]
         for name in ("ptir_create", "ptir_destroy", "ptir_build_bvh", "ptir_trace", "ptir_trace_bwd",
                      "ptir_cache_create", "ptir_cache_destroy", "ptir_cache_read_params", "ptir_cache_write_params",
                      "ptir_cache_read_grads", "ptir_cache_query", "ptir_cache_backprop", "ptir_cache_step", "ptir_cache_train",
                      "ptir_env_table_build", "ptir_env_table_build_noalloc", "ptir_camera_rays", "ptir_resolve_forward",
-                     "ptir_resolve_backward"):
+                     "ptir_resolve_backward", "ptir_adam_step", "ptir_ragged_layout"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib

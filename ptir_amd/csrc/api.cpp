@@ -52,8 +52,12 @@ ptir_status check_scene(ptir_tracer* t, const ptir_scene* sc, const ptir_launch*
         return fail(t, PTIR_ERR_UNSUPPORTED, "point, sphere and mesh lights are not supported; pass empty light tensors");
     if (sc->num_gaussians && (!sc->particle_density || !sc->particle_material || !sc->particle_radiance))
         return fail(t, PTIR_ERR_INVALID_ARGUMENT, "null particle tensor");
-    if (la->B < 0 || la->H < 0 || la->W < 0) return fail(t, PTIR_ERR_INVALID_ARGUMENT, "negative ray shape");
-    const long long paths = (long long)la->B * la->H * la->W;
+    if (la->num_images < 0) return fail(t, PTIR_ERR_INVALID_ARGUMENT, "negative num_images");
+    if (la->num_images > 0 && (!la->images || la->num_tiles <= 0 || la->num_paths <= 0))
+        return fail(t, PTIR_ERR_INVALID_ARGUMENT, "a ragged batch needs its image table and positive tile and path counts");
+    if (la->num_images == 0 && (la->B < 0 || la->H < 0 || la->W < 0))
+        return fail(t, PTIR_ERR_INVALID_ARGUMENT, "negative ray shape");
+    const long long paths = la->num_images > 0 ? (long long)la->num_paths : (long long)la->B * la->H * la->W;
     if (paths > 0 && (!la->ray_ori || !la->ray_dir || !la->ray_to_world))
         return fail(t, PTIR_ERR_INVALID_ARGUMENT, "null ray tensor");
     if (paths > (1ll << 31) - 1) return fail(t, PTIR_ERR_INVALID_ARGUMENT, "more than 2^31-1 paths in one launch");
@@ -107,9 +111,21 @@ ptir::DeviceScene make_scene(ptir_tracer* t, const ptir_scene* sc, const ptir_la
     S.render_opts = la->render_opts;
     S.cache_mode = la->cache_mode;
     S.frame = la->frame;
-    S.B = la->B;
-    S.H = la->H;
-    S.W = la->W;
+    if (la->num_images > 0) {
+        S.B = 1;
+        S.H = 1;
+        S.W = la->num_paths;
+        S.images = reinterpret_cast<const int4*>(la->images);
+        S.num_images = la->num_images;
+        S.num_blocks = la->num_tiles;
+    } else {
+        S.B = la->B;
+        S.H = la->H;
+        S.W = la->W;
+        S.images = nullptr;
+        S.num_images = 0;
+    }
+    S.rr_start = la->rr_start;
     S.enable_normals = t->opts.enable_normals;
     S.enable_hitcounts = t->opts.enable_hitcounts;
     S.enable_mis = t->opts.enable_mis;
@@ -248,7 +264,7 @@ ptir_status ptir_trace(ptir_tracer* t, const ptir_scene* sc, const ptir_launch* 
         return fail(t, PTIR_ERR_INVALID_ARGUMENT, "cache_mode needs cache_records");
     if (la->cache_mode) o.cache = out->cache_records;
     if (out->tape) {
-        const uint64_t need = ptir::tape_bytes((uint64_t)la->B * la->H * la->W, S.max_bounces, out->tape_hits_per_path);
+        const uint64_t need = ptir::tape_bytes(uint64_t(S.B) * S.H * S.W, S.max_bounces, out->tape_hits_per_path);
         if (out->tape_hits_per_path == 0 || out->tape_hits_per_path > 0xFFFFu - ptir::kTapeChunk ||
             out->tape_bytes < need)
             return fail(t, PTIR_ERR_INVALID_ARGUMENT, "tape too small for this launch (" + std::to_string(out->tape_bytes) +
@@ -269,7 +285,7 @@ ptir_status ptir_trace_bwd(ptir_tracer* t, const ptir_scene* sc, const ptir_laun
     ptir_status st = check_scene(t, sc, la);
     if (st != PTIR_OK) return st;
     if (!in || !out) return fail(t, PTIR_ERR_INVALID_ARGUMENT, "null backward inputs or outputs");
-    const long long paths = (long long)la->B * la->H * la->W;
+    const long long paths = la->num_images > 0 ? (long long)la->num_paths : (long long)la->B * la->H * la->W;
     if (paths > 0 && (!in->pbr || !in->light || !in->pbr_grad))
         return fail(t, PTIR_ERR_INVALID_ARGUMENT, "null pbr, light or pbr_grad");
     const size_t N = sc->num_gaussians;
@@ -296,7 +312,7 @@ ptir_status ptir_trace_bwd(ptir_tracer* t, const ptir_scene* sc, const ptir_laun
     ptir::BwdIO io{in->pbr, in->light, in->pbr_grad, in->light_grad, in->material_grad, t->bvh.dmat_leaf,
                    sc->environment ? out->environment_grad : nullptr, ptir::Tape{nullptr, 0u, 0u}, t->bvh.tape_state, 0};
     if (in->tape) {
-        const uint64_t need = ptir::tape_bytes((uint64_t)la->B * la->H * la->W, S.max_bounces, in->tape_hits_per_path);
+        const uint64_t need = ptir::tape_bytes(uint64_t(S.B) * S.H * S.W, S.max_bounces, in->tape_hits_per_path);
         if (in->tape_hits_per_path == 0 || in->tape_hits_per_path > 0xFFFFu - ptir::kTapeChunk || in->tape_bytes < need)
             return fail(t, PTIR_ERR_INVALID_ARGUMENT, "tape too small for this launch");
         io.tape = ptir::Tape{static_cast<uint32_t*>(const_cast<void*>(in->tape)), in->tape_hits_per_path, S.max_bounces};
@@ -310,6 +326,27 @@ ptir_status ptir_trace_bwd(ptir_tracer* t, const ptir_scene* sc, const ptir_laun
         if (e != hipSuccess) return hip_fail(t, e, "scatter_material_grad");
     }
     return end_counters(t, counters, s);
+}
+
+ptir_status ptir_ragged_layout(const int32_t* hw, int32_t n, int32_t* table, int32_t* num_tiles, int32_t* num_paths) {
+    if (n <= 0 || !hw || !table || !num_tiles || !num_paths)
+        return fail(nullptr, PTIR_ERR_INVALID_ARGUMENT, "ragged layout: bad arguments");
+    long long tiles = 0, paths = 0;
+    for (int i = 0; i < n; ++i) {
+        const int32_t h = hw[2 * i], w = hw[2 * i + 1];
+        if (h <= 0 || w <= 0) return fail(nullptr, PTIR_ERR_INVALID_ARGUMENT, "ragged layout: image sizes must be positive");
+        table[4 * i] = h;
+        table[4 * i + 1] = w;
+        table[4 * i + 2] = int32_t(tiles);
+        table[4 * i + 3] = int32_t(paths);
+        tiles += (long long)((w + 15) / 16) * ((h + 7) / 8);
+        paths += (long long)h * w;
+        if (tiles > (1ll << 31) - 1 || paths > (1ll << 31) - 1)
+            return fail(nullptr, PTIR_ERR_INVALID_ARGUMENT, "ragged layout: more than 2^31-1 paths or tiles");
+    }
+    *num_tiles = int32_t(tiles);
+    *num_paths = int32_t(paths);
+    return PTIR_OK;
 }
 
 uint64_t ptir_tape_bytes(int32_t B, int32_t H, int32_t W, uint32_t max_bounces, uint32_t hits_per_path) {

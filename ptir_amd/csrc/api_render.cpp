@@ -65,4 +65,22 @@ ptir_status ptir_resolve_backward(const ptir_resolve* r, const float* grad_dist,
     return hip_status(ptir::resolve_backward(io, g, static_cast<hipStream_t>(stream)), "resolve backward");
 }
 
+ptir_status ptir_adam_step(const ptir_adam_group* groups, int32_t num_groups, float beta1, float beta2, float eps,
+                           int32_t bias_correction, int32_t* step, void* stream) {
+    if (num_groups < 0 || num_groups > 64 || (num_groups && !groups) || !step)
+        return render_fail(PTIR_ERR_INVALID_ARGUMENT, "adam: bad group list or null step counter");
+    if (!(beta1 >= 0.f && beta1 < 1.f && beta2 >= 0.f && beta2 < 1.f && eps >= 0.f))
+        return render_fail(PTIR_ERR_INVALID_ARGUMENT, "adam: betas must be in [0,1) and eps >= 0");
+    ptir::AdamGroup g[64];
+    for (int k = 0; k < num_groups; ++k) {
+        const ptir_adam_group& a = groups[k];
+        if (a.rows && a.cols && (!a.param || !a.grad || !a.exp_avg || !a.exp_avg_sq))
+            return render_fail(PTIR_ERR_INVALID_ARGUMENT, "adam: null tensor in group " + std::to_string(k));
+        g[k] = ptir::AdamGroup{a.param, a.grad, a.exp_avg, a.exp_avg_sq, a.visibility, a.rows, a.cols, a.lr};
+    }
+    return hip_status(ptir::adam_step(g, num_groups, beta1, beta2, eps, bias_correction != 0, step,
+                                      static_cast<hipStream_t>(stream)),
+                      "adam step");
+}
+
 }  // extern "C"

@@ -119,6 +119,11 @@ struct DeviceScene {
     int32_t enable_normals, enable_hitcounts, enable_mis, enable_metallic, has_snormal;
     int32_t wave_traversal;  // primary rays walk the hierarchy per wave (wave_collect)
     int32_t tiles_x, tiles_y, num_blocks;  // launch tiling
+    // ragged batch: num_images > 0 images of their own size, table rows (H, W, first tile,
+    // first path); B, H, W are then (1, 1, total paths)
+    const int4* images;
+    int32_t num_images;
+    uint32_t rr_start;       // Russian roulette from this path vertex on (0 = off)
     unsigned long long* counters;          // work counter copies (see count()) or null
     int32_t* error_flag;
 };

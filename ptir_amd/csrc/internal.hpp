@@ -74,6 +74,19 @@ struct ResolveGrads {
     float *d_op, *d_dist, *d_pbr, *d_dir, *d_ind, *d_light, *d_nrm;
 };
 hipError_t resolve_forward(const ResolveIO& io, hipStream_t s);
+
+// Adam on optimised tensors (adam.hip).
+struct AdamGroup {
+    float* param;
+    const float* grad;
+    float *exp_avg, *exp_avg_sq;
+    const uint32_t* visibility;  // [rows] nonzero = update the row, or null
+    uint64_t rows;
+    uint32_t cols;
+    float lr;
+};
+hipError_t adam_step(const AdamGroup* groups, int n, float b1, float b2, float eps, int bias_correction, int32_t* step,
+                     hipStream_t s);
 hipError_t resolve_backward(const ResolveIO& io, const ResolveGrads& g, hipStream_t s);
 
 hipError_t launch_trace_fwd(const DeviceScene& S, const FwdOut& out, int kbuf, bool short_stack, hipStream_t s);

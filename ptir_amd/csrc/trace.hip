@@ -1199,6 +1199,19 @@ __device__ __forceinline__ void sample_next_bwd(const DeviceScene& S, Path& p, R
     }
 }
 
+// Russian roulette after the BRDF sample of path vertex numBounces (no counterpart in the
+// reference, which only cuts throughputs below 1e-4, referenceOptix.cu:62-65): from vertex
+// rr_start on the path survives with probability q = min(1, max throughput) and divides its
+// throughput by q.  One draw, taken identically by the forward and the replay backward
+// (which treats 1/q as a constant of the path).
+__device__ __forceinline__ bool russian_roulette(const DeviceScene& S, Path& p, Rng& rng) {
+    if (S.rr_start == 0u || p.numBounces < S.rr_start) return true;
+    const float q = fminf(1.0f, vmax(p.thr));
+    if (!(rng.next_1d() < q)) return false;
+    p.thr = p.thr * (1.0f / q);
+    return true;
+}
+
 // ---------------------------------------------------------------- tape records
 __device__ __forceinline__ void store_seg(uint32_t* rec, uint32_t s, const Seg& pl, const SegTape& tp) {
     float4* q = reinterpret_cast<float4*>(rec + kTapePathWords + kTapeSegWords * s);
@@ -1272,8 +1285,9 @@ __device__ __forceinline__ void cache_record(const DeviceScene& S, const Path& p
 
 // ---------------------------------------------------------------- launch mapping
 struct PixelId {
-    int x, y, z;
+    int x, y, z;  // launch index: pixel and image (the path's random stream)
     bool ok;
+    size_t pix;   // path index into the ray and output arrays
 };
 
 // Block b of the launch runs on XCD (b mod 8); give each XCD a contiguous range of tiles.
@@ -1283,16 +1297,36 @@ __device__ __forceinline__ PixelId pixel_of(const DeviceScene& S) {
     const int per = nb / 8, rem = nb % 8;
     const int xcd = b % 8, k = b / 8;
     const int tile = (xcd < rem) ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
+    const int lane = int(threadIdx.x) & 63, wave = int(threadIdx.x) >> 6;
+    PixelId p;
+    if (S.num_images > 0) {
+        // ragged batch: the image whose tile range holds this tile (last first-tile <= tile)
+        int lo = 0, hi = S.num_images - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (S.images[mid].z <= tile) lo = mid;
+            else hi = mid - 1;
+        }
+        const int4 im = S.images[lo];
+        const int tiles_x = (im.y + 15) / 16;
+        const int r = tile - im.z;
+        const int ty = r / tiles_x, tx = r - ty * tiles_x;
+        p.x = tx * 16 + wave * 8 + (lane & 7);
+        p.y = ty * 8 + (lane >> 3);
+        p.z = lo;
+        p.ok = p.x < im.y && p.y < im.x;
+        p.pix = size_t(im.w) + size_t(p.y) * im.y + p.x;
+        return p;
+    }
     const int tpi = S.tiles_x * S.tiles_y;
     const int z = tile / tpi;
     const int r = tile - z * tpi;
     const int ty = r / S.tiles_x, tx = r - ty * S.tiles_x;
-    const int lane = int(threadIdx.x) & 63, wave = int(threadIdx.x) >> 6;
-    PixelId p;
     p.x = tx * 16 + wave * 8 + (lane & 7);
     p.y = ty * 8 + (lane >> 3);
     p.z = z;
     p.ok = p.x < S.W && p.y < S.H && z < S.B;
+    p.pix = (size_t(z) * S.H + p.y) * S.W + p.x;
     return p;
 }
 
@@ -1340,7 +1374,7 @@ k_trace_fwd(DeviceScene S, FwdOut out,
     uint32_t* wstack = wst + (threadIdx.x / 64) * kStackDeep;
     const PixelId px = pixel_of(S);
     if (!px.ok) return;
-    const size_t pix = (size_t(px.z) * S.H + px.y) * S.W + px.x;
+    const size_t pix = px.pix;
     const size_t paths = size_t(S.B) * S.H * S.W;
     const bool rec = out.tape.base != nullptr;
     uint32_t* prec = rec ? out.tape.path(pix) : nullptr;
@@ -1404,6 +1438,7 @@ k_trace_fwd(DeviceScene S, FwdOut out,
             }
             sample_next(S, p, rng);
             if (vmax(p.thr) < 1e-4f) break;
+            if (!russian_roulette(S, p, rng)) break;
             if (rec) seg_tape_begin(tp, used);
             ray_intersect<K, SD>(S, p.cur.ray, p.cur, rng, true, false, true, out.vis, L, tp);
             if (rec) {
@@ -1442,7 +1477,7 @@ k_trace_bwd(DeviceScene S, BwdIO io) {
     const Lane L{lds + threadIdx.x};
     const PixelId px = pixel_of(S);
     if (!px.ok) return;
-    const size_t pix = (size_t(px.z) * S.H + px.y) * S.W + px.x;
+    const size_t pix = px.pix;
     const size_t paths = size_t(S.B) * S.H * S.W;
     const uint32_t* prec = io.tape.base ? io.tape.path(pix) : nullptr;
     if constexpr (TAPE) {
@@ -1505,6 +1540,7 @@ k_trace_bwd(DeviceScene S, BwdIO io) {
         if (S.cache_mode == 1u && p.numBounces == 2u) break;  // the forward ended here with a cache query
         sample_next_bwd<K, SD, TAPE>(S, p, rng, f);
         if (vmax(p.thr) < 1e-4f) break;
+        if (!russian_roulette(S, p, rng)) break;
         if constexpr (TAPE) {
             p.ccnt = load_seg(prec, p.numBounces, p.cur.ray, p.cur, p.cseg);
             count(S, C_SEGMENTS);
@@ -1518,6 +1554,7 @@ k_trace_bwd(DeviceScene S, BwdIO io) {
 }
 
 static void tiling(DeviceScene& S) {
+    if (S.num_images > 0) return;  // ragged: num_blocks is the table's tile count
     S.tiles_x = (S.W + 15) / 16;
     S.tiles_y = (S.H + 7) / 8;
     S.num_blocks = S.tiles_x * S.tiles_y * S.B;

@@ -115,9 +115,11 @@ def pack_alias_table(table: AliasTable | None, device="cpu") -> torch.Tensor:
     return torch.stack([p.reshape(table.height, table.width).float() for p in planes]).to(device).contiguous()
 
 
-def build_cdf_table(environment: torch.Tensor, grid=TABLE_GRID, out: torch.Tensor | None = None) -> torch.Tensor:
+def build_cdf_table(environment: torch.Tensor, grid=TABLE_GRID, out: torch.Tensor | None = None,
+                    allocate: bool = True) -> torch.Tensor:
     """[3, h, w] CDF-layout table of a device environment [H, W, 4], built on the GPU on the
-    current stream (ptir_env_table_build)."""
+    current stream (ptir_env_table_build).  allocate=False uses the allocation-free entry
+    point (tables up to 131072 cells), for code under graph capture."""
     from . import _lib
 
     env = environment
@@ -132,6 +134,6 @@ def build_cdf_table(environment: torch.Tensor, grid=TABLE_GRID, out: torch.Tenso
     elif tuple(out.shape) != (3, h, w) or out.dtype != torch.float32 or not out.is_contiguous():
         raise ValueError("out must be a contiguous float32 [3, h, w] tensor")
     stream = torch.cuda.current_stream(env.device).cuda_stream
-    _lib.check(_lib.lib().ptir_env_table_build(env.data_ptr(), env.shape[0], env.shape[1], h, w, out.data_ptr(),
-                                               C.c_void_p(stream)), None)
+    fn = _lib.lib().ptir_env_table_build if allocate else _lib.lib().ptir_env_table_build_noalloc
+    _lib.check(fn(env.data_ptr(), env.shape[0], env.shape[1], h, w, out.data_ptr(), C.c_void_p(stream)), None)
     return out

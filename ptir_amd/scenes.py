@@ -16,9 +16,9 @@ Workloads:
   axis on the surface normal, 8 material clusters, env 512x256 with a 500x sun lobe,
   4 views 800x800 at radius 3, 1 spp, maxBounces 3.
 * ``batched``: 500,000 Gaussians on the same shell, 16 views with ragged resolutions
-  and spp, env 1024x512, maxBounces 4.
+  and spp in one launch, env 1024x512, maxBounces 4 with Russian roulette from bounce 2.
 * ``scene``  : 3,000,000 Gaussians (ground clusters, box clusters, background shell),
-  env 2048x1024, 8 views 1280x720, maxBounces 5.
+  env 2048x1024, 8 views 1280x720, maxBounces 5 with Russian roulette from bounce 2.
 
 The `maxBounces` convention is the reference's: path vertices including the primary hit
 (src/kernels/cuda/referenceOptix.cu:52-67).
@@ -361,7 +361,120 @@ def target_image(scene: SyntheticScene, seed: int = 1) -> torch.Tensor:
     return torch.rand(*scene.rays_dir.shape[:3], 3, generator=g)
 
 
+def batched_scene(seed: int = 0, n: int = 500_000, n_views: int = 16, res_choices=(256, 512, 768, 1024),
+                  spp_choices=(1, 2, 4, 8), env_hw=(512, 1024), max_bounces: int = 4, rr_start: int = 2,
+                  res_scale: float = 1.0) -> SyntheticScene:
+    """Ragged batch: 500k Gaussians on the sphere-torus shell, 16 views on a sphere of radius
+    3, each with its own square resolution and samples per pixel; up to 3 bounces with
+    Russian roulette from bounce 2.  Every (view, sample) is one image of a ragged launch:
+    rays are flat world-space [1, 1, P, 3] in image order, extra["images"] lists the (H, W)
+    of each image and extra["view_of_image"] / extra["spp"] the view it belongs to and that
+    view's samples per pixel.  res_scale shrinks every resolution (parity-test sizes)."""
+    g = _gen(seed)
+    pos, rot, scale, dns, sh, nrm, albedo, rough, metal = _shell_gaussians(n, g)
+    env, sun = sun_sky_env(env_hw[0], env_hw[1], g)
+    cams = orbit_cameras(n_views, 3.0, g)
+    res = [res_choices[int(i)] for i in torch.randint(len(res_choices), (n_views,), generator=g)]
+    spps = [spp_choices[int(i)] for i in torch.randint(len(spp_choices), (n_views,), generator=g)]
+    oris, dirs, images, view_of = [], [], [], []
+    for v, (c, r, sp) in enumerate(zip(cams, res, spps)):
+        r = max(8, int(round(r * res_scale)))
+        o, d = world_rays([c], r, r, 40.0, sp)  # [sp, r, r, 3]
+        for k in range(sp):
+            oris.append(o[k].reshape(-1, 3))
+            dirs.append(d[k].reshape(-1, 3))
+            images.append((r, r))
+            view_of.append(v)
+    ori = torch.cat(oris).reshape(1, 1, -1, 3).contiguous()
+    dirv = torch.cat(dirs).reshape(1, 1, -1, 3).contiguous()
+    scene = SyntheticScene("batched", pos, rot, scale, dns, sh, nrm, albedo, rough, metal, env, ori, dirv,
+                           torch.eye(4).unsqueeze(0), sph_degree=3, max_bounces=max_bounces, spp=1)
+    scene.extra.update(sun=sun, images=images, view_of_image=view_of, spp=spps, rr_start=rr_start)
+    return scene
+
+
+def _box_surface(m: int, lo: torch.Tensor, hi: torch.Tensor, g: torch.Generator):
+    """m points on the surface of the box [lo, hi] (area-weighted faces) and their normals."""
+    ext = hi - lo
+    areas = torch.stack([ext[1] * ext[2], ext[1] * ext[2], ext[0] * ext[2], ext[0] * ext[2], ext[0] * ext[1],
+                         ext[0] * ext[1]])
+    face = torch.multinomial(areas / areas.sum(), m, replacement=True, generator=g)
+    u = torch.rand(m, 3, generator=g, dtype=torch.float64)
+    p = lo + u * ext
+    axis = face // 2
+    side = face % 2
+    p[torch.arange(m), axis] = torch.where(side == 0, lo[axis], hi[axis])
+    nrm = torch.zeros(m, 3, dtype=torch.float64)
+    nrm[torch.arange(m), axis] = torch.where(side == 0, -1.0, 1.0).double()
+    return p, nrm
+
+
+def scene_scale(seed: int = 0, n: int = 3_000_000, res=(720, 1280), n_views: int = 8, env_hw=(1024, 2048),
+                max_bounces: int = 5, rr_start: int = 2) -> SyntheticScene:
+    """Scene-scale unbounded workload: 60% of the Gaussians in 200 Gaussian-mixture clusters
+    over a 40 x 40 ground area (z in [0, 2]), 25% on the faces of 50 boxes (walls and
+    furniture), 15% on a background shell of radius 50 with log-scales N(0, 0.5); env
+    2048x1024; a camera path of 8 views at 1280x720 moving through the scene at eye height,
+    1 spp, 4 bounces with Russian roulette from bounce 2."""
+    g = _gen(seed)
+    n_g, n_b = int(0.60 * n), int(0.25 * n)
+    n_s = n - n_g - n_b
+    # ground clusters
+    centers = torch.cat([torch.rand(200, 2, generator=g, dtype=torch.float64) * 40 - 20,
+                         torch.rand(200, 1, generator=g, dtype=torch.float64) * 2], 1)
+    spread = torch.stack([0.3 + 1.2 * torch.rand(200, generator=g, dtype=torch.float64)] * 2 +
+                         [0.1 + 0.4 * torch.rand(200, generator=g, dtype=torch.float64)], 1)
+    cl = torch.randint(200, (n_g,), generator=g)
+    pg = centers[cl] + spread[cl] * torch.randn(n_g, 3, generator=g, dtype=torch.float64)
+    pg[:, 2] = pg[:, 2].clamp(0.0, 2.0)
+    ng = random_unit_vectors(n_g, g).double()
+    # box clusters
+    bl = torch.cat([torch.rand(50, 2, generator=g, dtype=torch.float64) * 36 - 18,
+                    torch.zeros(50, 1, dtype=torch.float64)], 1)
+    be = torch.cat([0.5 + 4.5 * torch.rand(50, 2, generator=g, dtype=torch.float64),
+                    0.5 + 2.5 * torch.rand(50, 1, generator=g, dtype=torch.float64)], 1)
+    per = torch.full((50,), n_b // 50, dtype=torch.long)
+    per[: n_b - int(per.sum())] += 1
+    pb, nb = [], []
+    for k in range(50):
+        p_, n_ = _box_surface(int(per[k]), bl[k], bl[k] + be[k], g)
+        pb.append(p_)
+        nb.append(n_)
+    pb, nb = torch.cat(pb), torch.cat(nb)
+    # background shell
+    vs = random_unit_vectors(n_s, g).double()
+    ps = 50.0 * vs
+    pos = torch.cat([pg, pb, ps]).float()
+    nrm = torch.cat([ng, nb, -vs]).float()
+    ls = torch.cat([-3.5 + 0.4 * torch.randn(n_g, 3, generator=g), -3.8 + 0.3 * torch.randn(n_b, 3, generator=g),
+                    0.5 * torch.randn(n_s, 3, generator=g)])
+    ls, _ = torch.sort(ls, dim=1, descending=True)  # flattest axis on the surface normal
+    scale = torch.exp(ls)
+    rot = quaternion_from_matrix(frames_from_normals(nrm, g))
+    dns = torch.sigmoid(torch.randn(n, 1, generator=g))
+    albedo, rough, metal = _materials_uniform(n, g)
+    sh = 0.1 * torch.randn(n, 48, generator=g)
+    env, sun = sun_sky_env(env_hw[0], env_hw[1], g, sun_gain=100.0)
+    oris, dirs = [], []
+    local = pinhole_rays(res[0], res[1], 60.0, 1)[0]
+    for v in range(n_views):
+        t = v / max(1, n_views - 1)
+        cam = torch.tensor([-12.0 + 24.0 * t, -6.0 + 4.0 * math.sin(3.0 * t), 1.6], dtype=torch.float64)
+        target = cam + torch.tensor([math.cos(0.8 * v), math.sin(0.8 * v), -0.15], dtype=torch.float64)
+        f = target - cam
+        R = look_at(-f / f.norm())  # look_at looks from its argument towards the origin
+        oris.append(cam.float().view(1, 1, 3).expand(res[0], res[1], 3))
+        dirs.append(torch.einsum("ij,hwj->hwi", R, local).float())
+    scene = SyntheticScene("scene", pos, rot, scale, dns, sh, nrm, albedo, rough, metal, env,
+                           torch.stack(oris).contiguous(), torch.stack(dirs).contiguous(), torch.eye(4).unsqueeze(0),
+                           sph_degree=3, max_bounces=max_bounces, spp=1)
+    scene.extra.update(sun=sun, rr_start=rr_start)
+    return scene
+
+
 WORKLOADS = {
     "small": small_scene,
     "object": object_scene,
+    "batched": batched_scene,
+    "scene": scene_scale,
 }

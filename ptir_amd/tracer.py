@@ -47,6 +47,41 @@ def _need(t: torch.Tensor, name: str, shape_tail=None, dtype=torch.float32) -> t
     return t.contiguous()
 
 
+@dataclass
+class RaggedBatch:
+    """Layout of a ragged launch: images of their own sizes traced together (several views
+    at their own resolutions and samples per pixel).  Rays and outputs of such a launch are
+    flat, [num_paths, C] in image order; `split` cuts a flat output back into images."""
+
+    hw: list
+    table: torch.Tensor  # device int32 [n, 4]: H, W, first tile, first path
+    num_tiles: int
+    num_paths: int
+
+    @classmethod
+    def build(cls, hw, device) -> "RaggedBatch":
+        hw = [(int(h), int(w)) for h, w in hw]
+        flat = (C.c_int32 * (2 * len(hw)))(*[v for p in hw for v in p])
+        table = (C.c_int32 * (4 * len(hw)))()
+        tiles, paths = C.c_int32(), C.c_int32()
+        _lib.check(_lib.lib().ptir_ragged_layout(flat, len(hw), table, C.byref(tiles), C.byref(paths)), None)
+        t = torch.tensor(list(table), dtype=torch.int32).view(len(hw), 4).to(device)
+        return cls(hw, t, int(tiles.value), int(paths.value))
+
+    def offsets(self):
+        return [int(v) for v in self.table[:, 3].cpu()]
+
+    def split(self, flat: torch.Tensor):
+        """[num_paths, C...] (or [1,1,num_paths,C...]) -> list of [H, W, C...] views."""
+        if flat.dim() >= 3 and tuple(flat.shape[:2]) == (1, 1):
+            flat = flat.reshape(self.num_paths, *flat.shape[3:])
+        out, start = [], 0
+        for h, w in self.hw:
+            out.append(flat[start:start + h * w].reshape(h, w, *flat.shape[1:]))
+            start += h * w
+        return out
+
+
 class HipTracer:
     """Drop-in for the reference's `OptixTracer` pybind class."""
 
@@ -91,8 +126,9 @@ class HipTracer:
         return int(_lib.lib().ptir_tape_bytes(B, H, W, max_bounces, self.tape_hits_per_path))
 
     def new_tape(self, ray_dir: torch.Tensor, max_bounces: int) -> torch.Tensor:
-        """Device buffer for the path tape of one trace call on rays of ray_dir's shape."""
-        B, H, W = ray_dir.shape[:3]
+        """Device buffer for the path tape of one trace call on rays of ray_dir's shape
+        ([B,H,W,3], or flat [P,3] for a ragged launch)."""
+        B, H, W = ray_dir.shape[:3] if ray_dir.dim() == 4 else (1, 1, ray_dir.numel() // 3)
         n = self.tape_bytes(B, H, W, int(max_bounces))
         if n == 0:
             raise ValueError("invalid launch size for a tape")
@@ -160,9 +196,13 @@ class HipTracer:
 
     @staticmethod
     def _launch(frame, ray_to_world, ray_ori, ray_dir, render_opts, sph_degree, min_transmittance, max_bounces,
-                cache_mode=0):
+                cache_mode=0, ragged: RaggedBatch | None = None, rr_start: int = 0):
         ro = _need(ray_ori, "ray_ori", (3,))
         rd = _need(ray_dir, "ray_dir", (3,))
+        if ragged is not None:
+            if ro.numel() != 3 * ragged.num_paths or rd.numel() != 3 * ragged.num_paths:
+                raise ValueError(f"a ragged launch takes {ragged.num_paths} rays, got {ro.numel() // 3}")
+            ro, rd = ro.reshape(1, 1, -1, 3), rd.reshape(1, 1, -1, 3)
         if ro.dim() != 4 or ro.shape != rd.shape:
             raise ValueError(f"rays must be [B,H,W,3] with equal shapes, got {tuple(ro.shape)} and {tuple(rd.shape)}")
         T = ray_to_world
@@ -177,6 +217,10 @@ class HipTracer:
         la.render_opts, la.sph_degree = int(render_opts), int(sph_degree)
         la.min_transmittance, la.max_bounces = float(min_transmittance), int(max_bounces)
         la.cache_mode = int(cache_mode)
+        la.rr_start = int(rr_start)
+        if ragged is not None:
+            la.num_images, la.images = len(ragged.hw), _ptr(ragged.table)
+            la.num_tiles, la.num_paths = ragged.num_tiles, ragged.num_paths
         return la, [ro, rd, T]
 
     # -------------------------------------------------------------- trace
@@ -185,18 +229,24 @@ class HipTracer:
               light_alias_table=None, mesh_light_vertices=None, mesh_light_triangles=None, mesh_lights=None,
               mesh_light_triangle_alias_table=None, render_opts=0, sph_degree=3, min_transmittance=1e-3,
               max_bounces=3, counters=False, tape: torch.Tensor | None = None, cache_mode: int = 0,
-              cache_records: torch.Tensor | None = None):
+              cache_records: torch.Tensor | None = None, ragged: RaggedBatch | None = None, rr_start: int = 0):
         """OptixTracer::trace (optixTracer.cpp:909-1013): returns the 13 outputs
         (rad, dns, hit[2], hit2, distortion, nrm, snrm, material, hits, visibility, pbr,
         light, pbr_components).
 
         cache_mode / cache_records (no reference counterpart): 1 ends every path at its first
         secondary vertex and writes a radiance-cache query there, 2 traces on and writes the
-        query with its training target; cache_records is [B,H,W,20] float32."""
+        query with its training target; cache_records is [B,H,W,20] float32.
+
+        ragged (no reference counterpart): trace the images of a RaggedBatch in one launch;
+        rays are flat [num_paths, 3] (world space) and the outputs [1, 1, num_paths, C].
+        rr_start (no reference counterpart): Russian roulette from that path vertex on."""
         sc, keep, _ = self._scene(particle_density, particle_material, particle_radiance, particle_shading_normal,
                                   environment, environment_alias_table, lights, light_alias_table, mesh_lights)
         la, keep2 = self._launch(frame_id, ray_to_world, ray_ori, ray_dir, render_opts, sph_degree,
-                                 min_transmittance, max_bounces, cache_mode)
+                                 min_transmittance, max_bounces, cache_mode, ragged, rr_start)
+        if ragged is not None:
+            keep2.append(ragged.table)
         B, H, W = la.B, la.H, la.W
         dev = ray_dir.device
         f = dict(dtype=torch.float32, device=dev)
@@ -240,7 +290,7 @@ class HipTracer:
                   mesh_light_triangle_alias_table, ray_rad_grd, ray_dns_grd, ray_hit_grd, ray_hit2_grd, ray_dist_grd,
                   ray_nrm_grd, ray_snrm_grd, ray_mat_grd, ray_pbr_grd, ray_light_grd, render_opts=0, sph_degree=3,
                   min_transmittance=1e-3, max_bounces=3, counters=False, tape: torch.Tensor | None = None,
-                  cache_mode: int = 0):
+                  cache_mode: int = 0, ragged: RaggedBatch | None = None, rr_start: int = 0):
         """OptixTracer::traceBwd (optixTracer.cpp:1015-1145): returns (dDensity [N,12],
         dMaterial [N,5], dSH, dShadingNormal [N,3], dEnv [H,W,4]).  cache_mode must be the
         forward's (with 1 the replay ends at the first secondary vertex, where the forward
@@ -248,7 +298,9 @@ class HipTracer:
         sc, keep, env = self._scene(particle_density, particle_material, particle_radiance, particle_shading_normal,
                                     environment, environment_alias_table, lights, light_alias_table, mesh_lights)
         la, keep2 = self._launch(frame_id, ray_to_world, ray_ori, ray_dir, render_opts, sph_degree,
-                                 min_transmittance, max_bounces, cache_mode)
+                                 min_transmittance, max_bounces, cache_mode, ragged, rr_start)
+        if ragged is not None:
+            keep2.append(ragged.table)
         B, H, W = la.B, la.H, la.W
 
         def pix(t, name, c, allow_none=False):

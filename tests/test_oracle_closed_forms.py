@@ -305,7 +305,8 @@ def test_indirect_option_under_an_emissive_ceiling_matches_quadrature():
     np.testing.assert_allclose(est[2], est[0], rtol=1e-12)           # all indirect
 
 
-def test_two_bounce_wall_and_floor_under_constant_sky_matches_quadrature():
+@pytest.mark.parametrize("rr_start", [0, 2])
+def test_two_bounce_wall_and_floor_under_constant_sky_matches_quadrature(rr_start):
     """maxBounces = 3 (two bounces): an opaque Lambert+GGX floor z = 0 (x > 0 side seen by
     the camera) meets an opaque wall x = 0 (normal +x), under a constant sky E.  Camera
     rays hit the floor at x0 > 0 head-on.  From the floor a direction escapes iff its x
@@ -319,7 +320,9 @@ def test_two_bounce_wall_and_floor_under_constant_sky_matches_quadrature():
     with the power-heuristic weights of the first bounce (w_b, w_l with p_l = 1/4pi for a
     constant sky), no MIS at the second bounce, the NEE term kept only when the BRDF sample
     after it survives (P_ok, environment.cuh:636-648) and a shadow ray toward the wall
-    passing with probability 1 - a (a = 0.99, stochastic occlusion, pathTracer.cuh:161-214)."""
+    passing with probability 1 - a (a = 0.99, stochastic occlusion, pathTracer.cuh:161-214).
+    With Russian roulette at the second vertex (rr_start = 2, an extension) the expectation
+    is unchanged: survival with probability q and division by q is unbiased."""
     E, albedo, rough = 1.3, 0.6, 1.0
     dens = _gaussians([_plane((0, 0, 0), 2), _plane((0, 0, 0), 0)])
     mat = torch.tensor([[albedo] * 3 + [rough, 0.0]] * 2)
@@ -330,7 +333,7 @@ def test_two_bounce_wall_and_floor_under_constant_sky_matches_quadrature():
     frames = 24
     acc = np.zeros((3, 3))
     for f in range(frames):
-        inp = O.Inputs(dens, mat, torch.zeros(2, 48), sn, env, tab, max_bounces=3, frame=f)
+        inp = O.Inputs(dens, mat, torch.zeros(2, 48), sn, env, tab, max_bounces=3, frame=f, rr_start=rr_start)
         out = O.trace(inp, rays, nthreads=8)
         acc[0] += out["pbr"].reshape(-1, 3).mean(0)
         acc[1] += out["comp"][..., 0, :].reshape(-1, 3).mean(0)
