@@ -32,6 +32,8 @@ class OracleScene(C.Structure):
         ("enable_mis", C.c_int32), ("enable_metallic", C.c_int32), ("render_opts", C.c_int32),
         ("min_transmittance", C.c_float), ("max_bounces", C.c_uint32), ("frame", C.c_uint32), ("kbuf", C.c_int32),
         ("tab_kind", C.c_int32), ("rr_start", C.c_uint32),
+        ("t_density", C.c_void_p), ("t_material", C.c_void_p), ("t_sh", C.c_void_p), ("t_snormal", C.c_void_p),
+        ("t_env", C.c_void_p),
     ]
 
 
@@ -100,6 +102,8 @@ def lib():
                                         C.c_void_p]
         L.oracle_candidates.restype = C.c_int
         L.oracle_scene_aabb.argtypes = [C.POINTER(OracleScene), C.c_void_p]
+        L.oracle_trace_dual.argtypes = [C.POINTER(OracleScene), C.POINTER(OracleRays), C.POINTER(OracleFwdOut),
+                                        C.POINTER(OracleFwdOut), C.c_int]
         _lib = L
     return _lib
 
@@ -187,6 +191,43 @@ def trace(inp: Inputs, rays: Rays, precision=64, nthreads=None, counters=False):
     if counters:
         out["counters"] = cnt.as_dict()
     return out
+
+
+TANGENT_FIELDS = {"density": "t_density", "material": "t_material", "sh": "t_sh", "snormal": "t_snormal",
+                  "env": "t_env"}
+
+
+def trace_dual(inp: Inputs, rays: Rays, tangents: dict, nthreads=None):
+    """FP64 forward on dual numbers: (outputs, derivatives of the outputs along the direction
+    `tangents` = {input name: array shaped like that input}, names of TANGENT_FIELDS)."""
+    keep = {}
+    for name, field in TANGENT_FIELDS.items():
+        t = tangents.get(name)
+        arr = None if t is None else _f32(t)
+        if arr is not None:
+            ref = getattr(inp, name if name != "env" else "env")
+            if ref is None or arr.size != ref.size:
+                raise ValueError(f"tangent {name} must match the input's shape")
+        keep[name] = arr
+        setattr(inp.s, field, _ptr(arr))
+    B, H, W = rays.shape
+    out = {k: np.zeros((B, H, W, c), dtype=np.float64) for k, c in _FWD_CH.items()}
+    der = {k: np.zeros((B, H, W, c), dtype=np.float64) for k, c in _FWD_CH.items()}
+    out["visibility"] = np.zeros(inp.s.n, dtype=np.float64)
+    out["margin"] = np.ones((B, H, W), dtype=np.float64)
+    o = OracleFwdOut(**{k: _ptr(out[k]) for k in _FWD_KEYS + ["margin"]})
+    d = OracleFwdOut(**{k: _ptr(der[k]) for k in _FWD_CH})
+    try:
+        rc = lib().oracle_trace_dual(C.byref(inp.s), C.byref(rays.r), C.byref(o), C.byref(d),
+                                     nthreads or os.cpu_count())
+    finally:
+        for field in TANGENT_FIELDS.values():
+            setattr(inp.s, field, None)
+    if rc != 0:
+        raise RuntimeError(f"oracle_trace_dual failed: {rc}")
+    for r in (out, der):
+        r["comp"] = r["comp"].reshape(B, H, W, 2, 3)
+    return out, der
 
 
 def trace_bwd(inp: Inputs, rays: Rays, pbr, light, pbr_grad, light_grad, mat_grad, precision=64, nthreads=None,

@@ -55,7 +55,93 @@
 #include <thread>
 #include <vector>
 
+// ---------------------------------------------------------------- forward-mode dual number
+// The oracle is instantiated on Dual to get exact directional derivatives of every output
+// along a chosen direction in parameter space (the "exact adjoint" of the GPU is checked
+// against them, and they against central differences).  Comparisons use the value only:
+// discrete decisions are those of the FP64 evaluation, and the derivative is that of the
+// smooth piece of the estimator the path lies on.
+struct Dual {
+    double v = 0.0, d = 0.0;
+    Dual() = default;
+    Dual(double x) : v(x), d(0.0) {}  // NOLINT: constants and inputs enter as duals with zero tangent
+    Dual(double x, double t) : v(x), d(t) {}
+    explicit operator double() const { return v; }
+    explicit operator float() const { return float(v); }
+    explicit operator int() const { return int(v); }
+    Dual operator-() const { return {-v, -d}; }
+    Dual& operator+=(const Dual& o) { v += o.v; d += o.d; return *this; }
+    Dual& operator-=(const Dual& o) { v -= o.v; d -= o.d; return *this; }
+    Dual& operator*=(const Dual& o) { d = d * o.v + v * o.d; v *= o.v; return *this; }
+    Dual& operator/=(const Dual& o) { d = (d * o.v - v * o.d) / (o.v * o.v); v /= o.v; return *this; }
+};
+inline Dual operator+(Dual a, const Dual& b) { return a += b; }
+inline Dual operator-(Dual a, const Dual& b) { return a -= b; }
+inline Dual operator*(Dual a, const Dual& b) { return a *= b; }
+inline Dual operator/(Dual a, const Dual& b) { return a /= b; }
+inline Dual operator+(Dual a, double b) { return a += Dual(b); }
+inline Dual operator-(Dual a, double b) { return a -= Dual(b); }
+inline Dual operator*(Dual a, double b) { return {a.v * b, a.d * b}; }
+inline Dual operator/(Dual a, double b) { return {a.v / b, a.d / b}; }
+inline Dual operator+(double a, const Dual& b) { return Dual(a) + b; }
+inline Dual operator-(double a, const Dual& b) { return Dual(a) - b; }
+inline Dual operator*(double a, const Dual& b) { return {a * b.v, a * b.d}; }
+inline Dual operator/(double a, const Dual& b) { return Dual(a) / b; }
+inline bool operator<(const Dual& a, const Dual& b) { return a.v < b.v; }
+inline bool operator>(const Dual& a, const Dual& b) { return a.v > b.v; }
+inline bool operator<=(const Dual& a, const Dual& b) { return a.v <= b.v; }
+inline bool operator>=(const Dual& a, const Dual& b) { return a.v >= b.v; }
+inline bool operator==(const Dual& a, const Dual& b) { return a.v == b.v; }
+inline bool operator!=(const Dual& a, const Dual& b) { return a.v != b.v; }
+inline bool operator<(const Dual& a, double b) { return a.v < b; }
+inline bool operator>(const Dual& a, double b) { return a.v > b; }
+inline bool operator<=(const Dual& a, double b) { return a.v <= b; }
+inline bool operator>=(const Dual& a, double b) { return a.v >= b; }
+inline bool operator==(const Dual& a, double b) { return a.v == b; }
+inline bool operator!=(const Dual& a, double b) { return a.v != b; }
+inline bool operator<(double a, const Dual& b) { return a < b.v; }
+inline bool operator>(double a, const Dual& b) { return a > b.v; }
+inline double tangent_of(const Dual& x) { return x.d; }
+inline double tangent_of(double) { return 0.0; }
+inline double tangent_of(float) { return 0.0; }
+
+namespace std {
+inline Dual sqrt(const Dual& x) { const double r = std::sqrt(x.v); return {r, r > 0 ? x.d / (2 * r) : 0.0}; }
+inline Dual exp(const Dual& x) { const double e = std::exp(x.v); return {e, e * x.d}; }
+inline Dual log(const Dual& x) { return {std::log(x.v), x.d / x.v}; }
+inline Dual sin(const Dual& x) { return {std::sin(x.v), std::cos(x.v) * x.d}; }
+inline Dual cos(const Dual& x) { return {std::cos(x.v), -std::sin(x.v) * x.d}; }
+inline Dual fabs(const Dual& x) { return x.v < 0 ? -x : x; }
+inline Dual floor(const Dual& x) { return {std::floor(x.v), 0.0}; }
+inline Dual round(const Dual& x) { return {std::round(x.v), 0.0}; }
+inline Dual fmax(const Dual& a, const Dual& b) { return a.v >= b.v ? a : b; }
+inline Dual fmin(const Dual& a, const Dual& b) { return a.v <= b.v ? a : b; }
+inline Dual copysign(const Dual& a, const Dual& b) { return (a.v < 0) != (b.v < 0) ? -a : a; }
+inline Dual atan2(const Dual& y, const Dual& x) {
+    const double r2 = x.v * x.v + y.v * y.v;
+    return {std::atan2(y.v, x.v), (x.v * y.d - y.v * x.d) / r2};
+}
+inline Dual asin(const Dual& x) { return {std::asin(x.v), x.d / std::sqrt(std::max(1.0 - x.v * x.v, 1e-300))}; }
+inline Dual pow(const Dual& a, const Dual& b) {
+    const double p = std::pow(a.v, b.v);
+    return {p, p * (b.d * std::log(a.v) + b.v * a.d / a.v)};
+}
+template <> struct numeric_limits<Dual> {
+    static Dual infinity() { return Dual(std::numeric_limits<double>::infinity()); }
+};
+}  // namespace std
+
 namespace orc {
+
+// Input value with its tangent (zero unless the input array has a tangent array).
+template <class R> static inline R input(float v, const float* tangent, size_t i) {
+    (void)tangent;
+    (void)i;
+    return R(v);
+}
+template <> inline Dual input<Dual>(float v, const float* tangent, size_t i) {
+    return Dual(double(v), tangent ? double(tangent[i]) : 0.0);
+}
 
 // ---------------------------------------------------------------- vectors
 template <class R>
@@ -171,6 +257,13 @@ struct OracleScene {
     int32_t kbuf;            // hits per chunk (16 in the reference)
     int32_t tab_kind;        // 0: alias table (prob, alias, pdf); 1: CDF table (conditional, marginal, pdf)
     uint32_t rr_start;       // Russian roulette from this path vertex on (0 = off); no reference counterpart
+    // tangents of the inputs (optional, same shapes): the dual-number run differentiates
+    // every output along this direction
+    const float* t_density;
+    const float* t_material;
+    const float* t_sh;
+    const float* t_snormal;
+    const float* t_env;
 };
 
 struct OracleRays {
@@ -336,15 +429,16 @@ static void build_scene(Scene<R>& sc, const OracleScene* in) {
     sc.hi = v3s<R>(-std::numeric_limits<R>::infinity());
     for (int i = 0; i < n; ++i) {
         const float* p = in->density + 12 * i;
+        auto P = [&](int k) { return input<R>(p[k], in->t_density, 12 * size_t(i) + k); };
         Gauss<R>& G = sc.g[i];
-        G.mu = v3<R>(p[0], p[1], p[2]);
-        G.sigma = p[3];
-        const R w = p[4], x = p[5], y = p[6], z = p[7];
+        G.mu = v3<R>(P(0), P(1), P(2));
+        G.sigma = P(3);
+        const R w = P(4), x = P(5), y = P(6), z = P(7);
         // gaussianParticles.cuh:26-46: standard rotation matrix of quaternion (w,x,y,z)
         G.rot[0][0] = 1 - 2 * (y * y + z * z); G.rot[0][1] = 2 * (x * y - w * z); G.rot[0][2] = 2 * (x * z + w * y);
         G.rot[1][0] = 2 * (x * y + w * z); G.rot[1][1] = 1 - 2 * (x * x + z * z); G.rot[1][2] = 2 * (y * z - w * x);
         G.rot[2][0] = 2 * (x * z - w * y); G.rot[2][1] = 2 * (y * z + w * x); G.rot[2][2] = 1 - 2 * (x * x + y * y);
-        G.s = v3<R>(p[8], p[9], p[10]);
+        G.s = v3<R>(P(8), P(9), P(10));
         G.is = v3<R>(R(1) / G.s.x, R(1) / G.s.y, R(1) / G.s.z);
         const R ks = kernel_scale<R>(G.sigma, in->min_response, in->density_clamping != 0, in->kernel_degree);
         G.kscl = G.s * ks;
@@ -502,6 +596,7 @@ struct Tracer {
     double* dmat = nullptr;
     double* denv = nullptr;
     double* visibility = nullptr;
+    const OracleFwdOut* tan_out = nullptr;  // tangents of the outputs (dual-number runs)
     // decision margin of the path being traced (see OracleFwdOut::margin)
     mutable double margin = 1.0;
     void note(double a, double b) const { note_rel(&margin, a, b); }
@@ -546,7 +641,11 @@ struct Tracer {
 
     V3<R> sh_radiance(uint32_t id, V3<R> d) const {
         const float* c = in.sh + size_t(in.sh_stride) * id;
-        auto C = [&](int k) { return v3<R>(c[3 * k], c[3 * k + 1], c[3 * k + 2]); };
+        const size_t cb = size_t(in.sh_stride) * id;
+        auto C = [&](int k) {
+            return v3<R>(input<R>(c[3 * k], in.t_sh, cb + 3 * k), input<R>(c[3 * k + 1], in.t_sh, cb + 3 * k + 1),
+                         input<R>(c[3 * k + 2], in.t_sh, cb + 3 * k + 2));
+        };
         // gaussianParticles.cuh:48-87 real SH up to degree 3
         const R C0 = 0.28209479177387814, C1 = 0.4886025119029199;
         const R C2[] = {1.0925484305920792, -1.0925484305920792, 0.31539156525252005, -1.0925484305920792, 0.5462742152960396};
@@ -610,7 +709,9 @@ struct Tracer {
                 const bool acc = a > 0;
                 if (acc) {
                     const Gauss<R>& G = sc.g[id];
-                    const float* mp = in.material + 5 * size_t(id);
+                    const float* mp0 = in.material + 5 * size_t(id);
+                    R mp[5];
+                    for (int q = 0; q < 5; ++q) mp[q] = input<R>(mp0[q], in.t_material, 5 * size_t(id) + q);
                     const R w = a * pl.T;
                     const R proj = dot(grd, -gro);
                     const V3<R> grds = G.s * grd * proj;
@@ -624,7 +725,10 @@ struct Tracer {
                     pl.hitDist2 += hitT * hitT * w;
                     if (in.enable_normals) {
                         const float* sn = sn_of(id);
-                        if (sn) snAcc += w * v3<R>(sn[0], sn[1], sn[2]);
+                        if (sn)
+                            snAcc += w * v3<R>(input<R>(sn[0], in.t_snormal, 3 * size_t(id)),
+                                               input<R>(sn[1], in.t_snormal, 3 * size_t(id) + 1),
+                                               input<R>(sn[2], in.t_snormal, 3 * size_t(id) + 2));
                         // (R s): the reference multiplies the rotation by the scale vector
                         const V3<R> rs = v3<R>(G.rot[0][0] * G.s.x + G.rot[0][1] * G.s.y + G.rot[0][2] * G.s.z,
                                                G.rot[1][0] * G.s.x + G.rot[1][1] * G.s.y + G.rot[1][2] * G.s.z,
@@ -732,8 +836,9 @@ struct Tracer {
         return f;
     }
     V3<R> texel(int x, int y) const {
-        const float* t = in.env + 4 * (size_t(y) * in.env_w + x);
-        return v3<R>(t[0], t[1], t[2]);
+        const size_t o = 4 * (size_t(y) * in.env_w + x);
+        const float* t = in.env + o;
+        return v3<R>(input<R>(t[0], in.t_env, o), input<R>(t[1], in.t_env, o + 1), input<R>(t[2], in.t_env, o + 2));
     }
     V3<R> bilinear(const Foot& f) const {
         return texel(f.x0, f.y0) * f.w00 + texel(f.x1, f.y0) * f.w10 + texel(f.x0, f.y1) * f.w01 + texel(f.x1, f.y1) * f.w11;
@@ -1313,19 +1418,23 @@ struct Tracer {
         margin = 1.0;
         ray_intersect(primary_ray(r, pix), p.cur, rng, false, true);
         const Seg<R>& c = p.cur;
-        auto put = [&](double* buf, int C, int k, R v) { if (buf) buf[pix * C + k] = double(v); };
-        put(out.rad, 3, 0, c.radiance.x); put(out.rad, 3, 1, c.radiance.y); put(out.rad, 3, 2, c.radiance.z);
-        put(out.dns, 1, 0, R(1) - c.T);
-        put(out.hit, 2, 0, c.hitDist); put(out.hit, 2, 1, c.lastHit);
-        put(out.hit2, 1, 0, c.hitDist2);
-        put(out.distortion, 1, 0, c.distortion);
+        auto put = [&](double* OracleFwdOut::*field, int C, int k, R v) {
+            if (double* buf = out.*field) buf[pix * C + k] = double(v);
+            if (tan_out)
+                if (double* tb = tan_out->*field) tb[pix * C + k] = tangent_of(v);
+        };
+        put(&OracleFwdOut::rad, 3, 0, c.radiance.x); put(&OracleFwdOut::rad, 3, 1, c.radiance.y); put(&OracleFwdOut::rad, 3, 2, c.radiance.z);
+        put(&OracleFwdOut::dns, 1, 0, R(1) - c.T);
+        put(&OracleFwdOut::hit, 2, 0, c.hitDist); put(&OracleFwdOut::hit, 2, 1, c.lastHit);
+        put(&OracleFwdOut::hit2, 1, 0, c.hitDist2);
+        put(&OracleFwdOut::distortion, 1, 0, c.distortion);
         if (in.enable_normals) {
-            put(out.nrm, 3, 0, c.normal.x); put(out.nrm, 3, 1, c.normal.y); put(out.nrm, 3, 2, c.normal.z);
-            put(out.snrm, 3, 0, c.inter.sn.x); put(out.snrm, 3, 1, c.inter.sn.y); put(out.snrm, 3, 2, c.inter.sn.z);
+            put(&OracleFwdOut::nrm, 3, 0, c.normal.x); put(&OracleFwdOut::nrm, 3, 1, c.normal.y); put(&OracleFwdOut::nrm, 3, 2, c.normal.z);
+            put(&OracleFwdOut::snrm, 3, 0, c.inter.sn.x); put(&OracleFwdOut::snrm, 3, 1, c.inter.sn.y); put(&OracleFwdOut::snrm, 3, 2, c.inter.sn.z);
         }
-        put(out.mat, 5, 0, c.inter.m.albedo.x); put(out.mat, 5, 1, c.inter.m.albedo.y); put(out.mat, 5, 2, c.inter.m.albedo.z);
-        put(out.mat, 5, 3, c.inter.m.rough); put(out.mat, 5, 4, c.inter.m.metal);
-        if (in.enable_hitcounts) put(out.hits, 1, 0, c.hits);
+        put(&OracleFwdOut::mat, 5, 0, c.inter.m.albedo.x); put(&OracleFwdOut::mat, 5, 1, c.inter.m.albedo.y); put(&OracleFwdOut::mat, 5, 2, c.inter.m.albedo.z);
+        put(&OracleFwdOut::mat, 5, 3, c.inter.m.rough); put(&OracleFwdOut::mat, 5, 4, c.inter.m.metal);
+        if (in.enable_hitcounts) put(&OracleFwdOut::hits, 1, 0, c.hits);
         if (!p.cur.inter.valid) {
             if (out.margin) out.margin[pix] = margin;
             return;
@@ -1343,10 +1452,10 @@ struct Tracer {
             ray_intersect(p.cur.ray, p.cur, rng, true, false);
         }
         if (out.margin) out.margin[pix] = margin;
-        put(out.pbr, 3, 0, p.L.x); put(out.pbr, 3, 1, p.L.y); put(out.pbr, 3, 2, p.L.z);
-        put(out.light, 3, 0, p.Lnb.x); put(out.light, 3, 1, p.Lnb.y); put(out.light, 3, 2, p.Lnb.z);
-        put(out.comp, 6, 0, p.direct.x); put(out.comp, 6, 1, p.direct.y); put(out.comp, 6, 2, p.direct.z);
-        put(out.comp, 6, 3, p.indirect.x); put(out.comp, 6, 4, p.indirect.y); put(out.comp, 6, 5, p.indirect.z);
+        put(&OracleFwdOut::pbr, 3, 0, p.L.x); put(&OracleFwdOut::pbr, 3, 1, p.L.y); put(&OracleFwdOut::pbr, 3, 2, p.L.z);
+        put(&OracleFwdOut::light, 3, 0, p.Lnb.x); put(&OracleFwdOut::light, 3, 1, p.Lnb.y); put(&OracleFwdOut::light, 3, 2, p.Lnb.z);
+        put(&OracleFwdOut::comp, 6, 0, p.direct.x); put(&OracleFwdOut::comp, 6, 1, p.direct.y); put(&OracleFwdOut::comp, 6, 2, p.direct.z);
+        put(&OracleFwdOut::comp, 6, 3, p.indirect.x); put(&OracleFwdOut::comp, 6, 4, p.indirect.y); put(&OracleFwdOut::comp, 6, 5, p.indirect.z);
     }
 
     // referenceBwdOptix.cu:27-74
@@ -1388,7 +1497,7 @@ struct Tracer {
 
 template <class R>
 static int run(const OracleScene* in, const OracleRays* rays, const OracleFwdOut* fwd, const OracleBwdIn* bi,
-               const OracleBwdOut* bo, int nthreads, OracleCounters* counters) {
+               const OracleBwdOut* bo, int nthreads, OracleCounters* counters, const OracleFwdOut* tangents = nullptr) {
     Scene<R> sc;
     build_scene(sc, in);
     const int64_t total = int64_t(rays->B) * rays->H * rays->W;
@@ -1405,6 +1514,7 @@ static int run(const OracleScene* in, const OracleRays* rays, const OracleFwdOut
     auto worker = [&](int tid) {
         Tracer<R> tr(sc);
         if (fwd) tr.visibility = fwd->visibility;  // idempotent writes of 1
+        tr.tan_out = tangents;
         if (bi && bo) {
             if (bo->dmat) { dm[tid].assign(size_t(n) * 5, 0.0); tr.dmat = dm[tid].data(); }
             if (bo->denv && envN) { de[tid].assign(envN, 0.0); tr.denv = de[tid].data(); }
@@ -1467,6 +1577,14 @@ int oracle_trace_bwd(const OracleScene* scene, const OracleRays* rays, const Ora
     if (!scene || !rays || !bin || !bout) return 1;
     if (precision == 32) return orc::run<float>(scene, rays, nullptr, bin, bout, nthreads, counters);
     return orc::run<double>(scene, rays, nullptr, bin, bout, nthreads, counters);
+}
+
+// Forward on dual numbers: the outputs in `out` and their derivatives along the direction
+// given by the scene's tangent arrays in `tangents` (same layout; visibility unused).
+int oracle_trace_dual(const OracleScene* scene, const OracleRays* rays, const OracleFwdOut* out,
+                      const OracleFwdOut* tangents, int nthreads) {
+    if (!scene || !rays || !out || !tangents) return 1;
+    return orc::run<Dual>(scene, rays, out, nullptr, nullptr, nthreads, nullptr, tangents);
 }
 
 // Building blocks exposed for the closed-form tests.
