@@ -4,6 +4,9 @@
 Metric: forward + backward path samples per second on the object-scale workload
 (300,000 Gaussians on a sphere-torus shell, env 512x256 with a sun lobe, 4 views of
 800x800, 1 spp, maxBounces 3 = primary + 2 secondary rays with NEE and MIS), FP32.
+The `nrc` block compares the radiance-cache form with the plain tracer at equal image MSE
+against a 1,024-spp FP64 oracle image; the `workloads` block times the whole inversion
+step (captured as one graph) on the batched ragged and the scene-scale workloads.
 One step is one training step of the tracer: BVH build, forward trace, L2 loss gradient
 against a seeded target, backward trace (material + environment gradients, path-replay
 semantics of the reference).  Inputs are synthetic and resident in HBM before timing.
@@ -87,6 +90,153 @@ def cpu_baseline(scene, table, stride: int) -> dict:
                       f"incl. the oracle's own BVH build, {dt:.1f} s"}
 
 
+def nrc_comparison(train_steps: int = 150, timed_steps: int = 5, spps=(1, 4, 16, 64)) -> dict:
+    """Radiance cache (NRC) form against the plain tracer on the object-scale workload.
+
+    Both render paths of up to maxBounces 4 (three bounces): the plain tracer traces them
+    out; the cache form ends them at the first secondary vertex with a cache query and trains
+    the cache online, one Adam step per training step, on up to 65,536 path suffixes from full
+    paths of every 4th pixel row.  Reported: fwd+bwd path samples/s of each form on the full
+    4 x 800 x 800 batch, the MSE of S-spp images of a 96x96 crop against a 1,024-spp FP64
+    oracle image of the plain tracer (tests/golden/c5_reference.npz), and, at equal MSE, the
+    samples and time each form needs."""
+    import numpy as np
+    import torch
+
+    from ptir_amd import envmap, scenes
+    from ptir_amd.cache import CachedPathTracer, RadianceCache
+    from ptir_amd.tracer import HipTracer
+
+    ref_path = os.path.join(ROOT, "tests", "golden", "c5_reference.npz")
+    ref = np.load(ref_path)
+    _, y0, x0, ch, cw = (int(v) for v in ref["crop"])
+    dev = torch.device("cuda:0")
+    sc = scenes.object_scene(0)
+    tab = envmap.pack_alias_table(envmap.build_environment_alias_table(sc.env)).to(dev)
+    g = sc.to(dev)
+    tr = HipTracer(enable_metallic=True)
+    tr.build_bvh(g.positions, g.rotation, g.scale, g.density)
+    scene_args = (g.particle_density(), g.particle_material(), g.features, g.shading_normal, g.env, tab)
+    box = torch.cat([g.positions.min(0).values - 0.05, g.positions.max(0).values + 0.05]).contiguous()
+    cached = CachedPathTracer(tr, RadianceCache(max_batch=65536, seed=0), box, max_bounces=4)
+    T = g.ray_to_world
+    for s in range(train_steps):
+        cached.train(100_000 + s, T, g.rays_ori, g.rays_dir, scene_args, offset=s % 4)
+    torch.cuda.synchronize()
+    _, _, loss = cached.cache.read_grads()
+    target = scenes.target_image(sc).to(dev)
+    tape = tr.new_tape(g.rays_dir, 4)
+
+    def bwd(outs, frame, cache_mode):
+        grad = ((outs[10] - target) * (2.0 / outs[10].numel())).contiguous()
+        tr.trace_bwd(frame, T, g.rays_ori, g.rays_dir, *outs[:8], outs[10], outs[11], *scene_args, None, None, None,
+                     None, None, None, None, None, None, None, None, None, None, None, grad, None, max_bounces=4,
+                     tape=tape, cache_mode=cache_mode)
+
+    def plain_step(f):
+        bwd(tr.trace(f, T, g.rays_ori, g.rays_dir, *scene_args, max_bounces=4, tape=tape), f, 0)
+
+    def nrc_step(f):
+        cached.train(200_000 + f, T, g.rays_ori, g.rays_dir, scene_args, offset=f % 4)
+        outs, _ = cached.render(f, T, g.rays_ori, g.rays_dir, scene_args, tape=tape)
+        bwd(outs, f, 1)
+
+    timing = {}
+    for name, fn in (("plain", plain_step), ("nrc", nrc_step)):
+        fn(0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for f in range(1, timed_steps + 1):
+            fn(f)
+        torch.cuda.synchronize()
+        timing[name] = (time.perf_counter() - t0) / timed_steps
+    o = g.rays_ori[:1, y0:y0 + ch, x0:x0 + cw].contiguous()
+    d = g.rays_dir[:1, y0:y0 + ch, x0:x0 + cw].contiguous()
+    refimg = torch.from_numpy(ref["image"]).double().to(dev)
+
+    def mse_curve(render):
+        acc = torch.zeros(ch, cw, 3, dtype=torch.float64, device=dev)
+        out, n = {}, 0
+        for S in spps:
+            while n < S:
+                # frames from 10,000 on: disjoint from the reference image's frames 0..1023
+                acc += render(10_000 + n)[0].double()
+                n += 1
+            out[S] = float(((acc / n - refimg) ** 2).mean())
+        return out
+
+    mse_plain = mse_curve(lambda f: tr.trace(f, T, o, d, *scene_args, max_bounces=4)[10])
+    mse_nrc = mse_curve(lambda f: cached.render(f, T, o, d, scene_args)[0][10])
+    mse_cut = mse_curve(lambda f: tr.trace(f, T, o, d, *scene_args, max_bounces=2)[10])
+    paths = sc.num_paths
+
+    def spp_for(mse_target):
+        """Samples the plain tracer needs for mse_target (log-log interpolation of its curve)."""
+        xs = np.log([float(S) for S in spps])
+        ys = np.log([mse_plain[S] for S in spps])
+        slope = (ys[-1] - ys[0]) / (xs[-1] - xs[0])
+        return float(np.exp(xs[0] + (np.log(mse_target) - ys[0]) / slope))
+
+    equal = {}
+    for S in spps:
+        sp = spp_for(mse_nrc[S])
+        equal[str(S)] = {"nrc_spp": S, "plain_spp_same_mse": round(sp, 2),
+                         "time_ratio_plain_over_nrc": round((sp * timing["plain"]) / (S * timing["nrc"]), 3)}
+    return {
+        "workload": "object-scale, maxBounces 4; cache queried at the first secondary vertex",
+        "plain": {"ms_per_step": round(timing["plain"] * 1e3, 3), "path_samples_per_s": round(paths / timing["plain"], 1),
+                  "mse_vs_fp64": {str(k): v for k, v in mse_plain.items()}},
+        "nrc": {"ms_per_step": round(timing["nrc"] * 1e3, 3), "path_samples_per_s": round(paths / timing["nrc"], 1),
+                "mse_vs_fp64": {str(k): v for k, v in mse_nrc.items()},
+                "train_steps_before_timing": train_steps, "train_samples_per_step": round(
+                    cached.train_samples / max(1, train_steps + timed_steps + 1), 1),
+                "last_train_loss": float(loss.item())},
+        "cut_at_first_secondary_vertex": {"mse_vs_fp64": {str(k): v for k, v in mse_cut.items()}},
+        "equal_mse": equal,
+        "reference": {"spp": int(ref["spp"]), "crop": [y0, x0, ch, cw], "precision": "fp64 oracle",
+                      "indirect_share": float(ref["indirect_share"])},
+    }
+
+
+def time_inversion_workload(name: str, steps: int, warmup: int) -> dict:
+    """Steps/s of the whole inversion step (device env table, BVH refit, forward, loss
+    gradient, backward, Adam) on a workload, captured as one graph and replayed."""
+    import torch
+
+    from ptir_amd import scenes
+    from ptir_amd.inversion import InversionStep
+    from ptir_amd.tracer import HipTracer, RaggedBatch
+
+    dev = torch.device("cuda:0")
+    t_gen = time.perf_counter()
+    sc = scenes.WORKLOADS[name](0)
+    gen_s = time.perf_counter() - t_gen
+    kw = {"rr_start": sc.extra.get("rr_start", 0)}
+    if name == "batched":
+        kw["ragged"] = RaggedBatch.build(sc.extra["images"], dev)
+    g = sc.to(dev)
+    target = scenes.target_image(sc).to(dev)
+    step = InversionStep(HipTracer(enable_metallic=True, env_sampling="cdf"), g, target, **kw)
+    step.capture()
+    for _ in range(warmup):
+        step.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step.replay()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    paths = sc.num_paths
+    out = {"gaussians": sc.num_gaussians, "paths_per_step": paths, "max_bounces": sc.max_bounces,
+           "rr_start": kw["rr_start"], "ms_per_step": round(dt * 1e3, 3), "path_samples_per_s": round(paths / dt, 1),
+           "steps": steps, "graph": True, "scene_generation_s": round(gen_s, 1), "loss_last": round(step.loss(), 6)}
+    if name == "batched":
+        out["images"] = len(sc.extra["images"])
+    del step, g
+    torch.cuda.empty_cache()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +246,9 @@ def main() -> int:
     ap.add_argument("--cpu-stride", type=int, default=8, help="oracle path sample stride for cpu_baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tape", action="store_true", help="backward re-traces every segment (reference behaviour)")
+    ap.add_argument("--no-nrc", action="store_true", help="skip the radiance-cache comparison")
+    ap.add_argument("--no-workloads", action="store_true",
+                    help="skip the batched (ragged, 500k) and scene-scale (3M) inversion-step timings")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only copy a float32 buffer of BYTES bytes 5 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -184,12 +337,25 @@ def main() -> int:
         "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "object-scale: 300k Gaussians, 4 views 800x800, 1 spp, maxBounces 3, env 512x256, "
-                               "plain tracer (radiance cache not built), ptir-mode backward",
+                               "plain tracer fwd+bwd (ptir-mode backward); radiance-cache comparison in nrc",
                    "gaussians": scene.num_gaussians, "paths_per_step": paths, "kbuffer": args.kbuffer,
                    "backward": "re-trace" if tape is None else "path tape"},
         "roofline": roofline,
         "work_units": {"fwd": cf, "bwd": cb},
     }
+    if not args.no_nrc:
+        del tape, tk
+        tape, tk = None, {}
+        torch.cuda.empty_cache()
+        print("radiance-cache comparison", file=sys.stderr, flush=True)
+        line["nrc"] = nrc_comparison()
+    if not args.no_workloads:
+        del tape, tk
+        torch.cuda.empty_cache()
+        line["workloads"] = {}
+        for name in ("batched", "scene"):
+            print(f"timing workload {name}", file=sys.stderr, flush=True)
+            line["workloads"][name] = time_inversion_workload(name, steps=3, warmup=1)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(scene, table, args.cpu_stride)
     print(json.dumps(line), flush=True)

@@ -94,7 +94,11 @@ typedef struct ptir_launch {
      * cache's answer is added by ptir_cache_query); 2 = trace the whole path and write the
      * query with its training target (the radiance the rest of the path carried, per unit
      * throughput).  1 and 2 need max_bounces >= 3 and ptir_fwd_outputs.cache_records.
-     * ptir_trace_bwd must be given the same value. */
+     * ptir_trace_bwd must be given the same value.  In mode 1 it expects the pbr after
+     * ptir_cache_query added the cache term: that term scales with the throughput of the
+     * first bounce, so it belongs to the radiance the first vertex's BRDF gradient
+     * multiplies; the cache's answer itself is held constant (no gradient flows into the
+     * cache or into the material seen at the cache vertex). */
     uint32_t cache_mode;
     /* Ragged batch (no counterpart in the reference, which traces one [B,H,W] batch under one
      * camera transform, optixTracer.cpp:962): num_images > 0 traces images of different
@@ -230,7 +234,7 @@ uint64_t ptir_tape_bytes(int32_t B, int32_t H, int32_t W, uint32_t max_bounces, 
 typedef struct ptir_cache ptir_cache;
 
 typedef struct ptir_cache_options {
-    uint64_t max_batch;      /* records per training call */
+    uint64_t max_batch;      /* records per training call, at most 2^20 */
     uint32_t seed;           /* initial weights and table */
     float learning_rate;     /* Adam */
     float beta1, beta2, epsilon;

@@ -118,3 +118,60 @@ class RadianceCache:
         box = _dev(scene_box, "scene_box").reshape(-1)
         _lib.check_cache(_lib.lib().ptir_cache_train(self._h, rec.data_ptr(), rec.numel() // REC_FLOATS,
                                                      box.data_ptr(), _stream()), self._h)
+
+
+class CachedPathTracer:
+    """The radiance-cache form of the path tracer (no counterpart in the reference).
+
+    Rendering traces every path to its first secondary vertex (cache_mode 1) and adds the
+    cache's answer there, weighted by the path throughput.  Training traces a subset of the
+    pixels as full paths (cache_mode 2, up to max_bounces vertices), keeps the records of
+    the paths that reached the cache vertex (each holds the query and the radiance the rest
+    of its path carried) and takes one Adam step on up to `train_batch` of them.  The plain
+    multi-bounce tracer remains the reference the cache is judged against."""
+
+    def __init__(self, tracer, cache: RadianceCache, scene_box: torch.Tensor, max_bounces: int = 4,
+                 train_batch: int = 65536, train_stride: int = 4):
+        if max_bounces < 3:
+            raise ValueError("the cache stands in for bounces beyond the first: max_bounces must be >= 3")
+        if train_batch > cache.max_batch:
+            raise ValueError("train_batch exceeds the cache's max_batch")
+        self.tracer = tracer
+        self.cache = cache
+        self.box = scene_box.contiguous()
+        self.max_bounces = int(max_bounces)
+        self.train_batch = int(train_batch)
+        self.train_stride = int(train_stride)
+        self.batch = None
+        self.train_paths = 0
+        self.train_samples = 0
+
+    def _records(self, ray_dir):
+        B, H, W = ray_dir.shape[:3]
+        return torch.zeros((B, H, W, REC_FLOATS), dtype=torch.float32, device=ray_dir.device)
+
+    def render(self, frame, ray_to_world, ray_ori, ray_dir, scene_args, **kw):
+        """Forward outputs of the cache form (the 13 tracer outputs; pbr and the indirect part of
+        pbr_components include the cache term) and the cache records of the launch."""
+        rec = self._records(ray_dir)
+        outs = self.tracer.trace(frame, ray_to_world, ray_ori, ray_dir, *scene_args, max_bounces=self.max_bounces,
+                                 cache_mode=1, cache_records=rec, **kw)
+        self.cache.query(rec.view(-1, REC_FLOATS), self.box, pbr=outs[10], pbr_components=outs[12])
+        return outs, rec
+
+    def train(self, frame, ray_to_world, ray_ori, ray_dir, scene_args, offset: int = 0, **kw) -> int:
+        """One online training step from full paths of every train_stride-th pixel row
+        (starting at offset).  Returns the number of training samples used."""
+        o = ray_ori[:, offset::self.train_stride].contiguous()
+        d = ray_dir[:, offset::self.train_stride].contiguous()
+        rec = self._records(d)
+        self.tracer.trace(frame, ray_to_world, o, d, *scene_args, max_bounces=self.max_bounces, cache_mode=2,
+                          cache_records=rec, **kw)
+        flat = rec.view(-1, REC_FLOATS)
+        live = flat[flat[:, 0] != 0.0][: self.train_batch].contiguous()
+        self.train_paths += flat.shape[0]
+        n = live.shape[0]
+        self.train_samples += n
+        if n:
+            self.cache.train(live, self.box)
+        return n

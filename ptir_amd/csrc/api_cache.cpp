@@ -39,8 +39,10 @@ extern "C" {
 ptir_status ptir_cache_create(const ptir_cache_options* o, ptir_cache** out, void* stream) {
     if (!o || !out) return cfail(nullptr, PTIR_ERR_INVALID_ARGUMENT, "null options or output handle");
     *out = nullptr;
-    if (o->max_batch == 0 || o->max_batch > (1ull << 26))
-        return cfail(nullptr, PTIR_ERR_INVALID_ARGUMENT, "max_batch must be in [1, 2^26]");
+    // scratch grows with max_batch: 68 KB of per-tile weight gradients per 64 records, so cap
+    // it at 2^20 records (1.1 GB)
+    if (o->max_batch == 0 || o->max_batch > (1ull << 20))
+        return cfail(nullptr, PTIR_ERR_INVALID_ARGUMENT, "max_batch must be in [1, 2^20]");
     if (!(o->learning_rate >= 0.f) || !(o->beta1 >= 0.f && o->beta1 < 1.f) || !(o->beta2 >= 0.f && o->beta2 < 1.f) ||
         !(o->epsilon > 0.f))
         return cfail(nullptr, PTIR_ERR_INVALID_ARGUMENT, "Adam needs lr >= 0, beta1 and beta2 in [0,1), epsilon > 0");

@@ -139,6 +139,7 @@ struct NrcBuffers {
     float* z = nullptr;             // [max_batch, 4] output pre-activations
     float* dx = nullptr;            // [max_batch, 32] hash-feature gradients
     float* dw_part = nullptr;       // [tiles, kParams] per-tile weight gradients
+    float* dw_chunks = nullptr;     // [ceil(tiles / 32), kParams] sums over blocks of 32 tiles
     float* loss = nullptr;          // [1]
     uint32_t* step = nullptr;       // [1] Adam step counter
 };

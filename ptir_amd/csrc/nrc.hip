@@ -226,7 +226,9 @@ __global__ void __launch_bounds__(kBlock) k_forward(FwdArgs a) {
         for (int c = 0; c < 4; ++c) {
             const uint64_t smp = tile * 64 + 16 * c + j;
             live[c] = smp < a.n;
-            if (live[c] && (TRAIN || a.rec[smp * kRecFloats + kRecFlag] != 0.f)) {
+            // records of paths that never reached the cache vertex (flag 0) carry no query:
+            // they enter as zeros, whatever the rest of their record holds
+            if (live[c] && a.rec[smp * kRecFloats + kRecFlag] != 0.f) {
                 const float* rec = a.rec + smp * kRecFloats;
                 B[c][0] = encode_chunk(rec, a.box, a.table, res, g, 0);
                 B[c][1] = encode_chunk(rec, a.box, a.table, res, g, 1);
@@ -445,12 +447,25 @@ __global__ void __launch_bounds__(kBlock) k_backward(BwdArgs a) {
     }
 }
 
-// dW = sum over tiles of the per-tile gradients (deterministic)
-__global__ void k_reduce(const float* __restrict__ part, uint64_t tiles, float* __restrict__ dw) {
+// dW = sum over tiles of the per-tile gradients, in two deterministic passes: blocks of
+// kReduceChunk tiles first (blockIdx.y = chunk), then the chunk sums.
+constexpr int kReduceChunk = 32;
+
+__global__ void k_reduce_chunks(const float* __restrict__ part, uint64_t tiles, float* __restrict__ sums) {
+    const int p = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (p >= kParams) return;
+    const uint64_t t0 = uint64_t(blockIdx.y) * kReduceChunk;
+    const uint64_t t1 = t0 + kReduceChunk < tiles ? t0 + kReduceChunk : tiles;
+    float s = 0.f;
+    for (uint64_t t = t0; t < t1; ++t) s += part[t * kParams + p];
+    sums[uint64_t(blockIdx.y) * kParams + p] = s;
+}
+
+__global__ void k_reduce(const float* __restrict__ sums, uint64_t chunks, float* __restrict__ dw) {
     const int p = int(blockIdx.x * blockDim.x + threadIdx.x);
     if (p >= kParams) return;
     float s = 0.f;
-    for (uint64_t t = 0; t < tiles; ++t) s += part[t * kParams + p];
+    for (uint64_t c = 0; c < chunks; ++c) s += sums[c * kParams + p];
     dw[p] = s;
 }
 
@@ -539,6 +554,7 @@ hipError_t nrc_alloc(NrcBuffers& b, uint64_t max_batch, uint32_t seed, hipStream
     if (e == hipSuccess) e = dalloc(b.z, max_batch * 4);
     if (e == hipSuccess) e = dalloc(b.dx, max_batch * 32);
     if (e == hipSuccess) e = dalloc(b.dw_part, tiles * kParams);
+    if (e == hipSuccess) e = dalloc(b.dw_chunks, ((tiles + kReduceChunk - 1) / kReduceChunk) * kParams);
     if (e == hipSuccess) e = dalloc(b.loss, 1);
     if (e == hipSuccess) e = dalloc(b.step, 1);
     if (e != hipSuccess) return e;
@@ -557,8 +573,8 @@ hipError_t nrc_alloc(NrcBuffers& b, uint64_t max_batch, uint32_t seed, hipStream
 }
 
 void nrc_free(NrcBuffers& b) {
-    void* ps[] = {b.w, b.dw, b.mw, b.vw, b.table, b.dtable, b.mt, b.vt, b.frags, b.act, b.z, b.dx, b.dw_part, b.loss,
-                  b.step};
+    void* ps[] = {b.w, b.dw, b.mw, b.vw, b.table, b.dtable, b.mt, b.vt, b.frags, b.act, b.z, b.dx, b.dw_part,
+                  b.dw_chunks, b.loss, b.step};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     b = NrcBuffers{};
@@ -603,7 +619,10 @@ hipError_t nrc_backprop(NrcBuffers& b, const float* rec, uint64_t n, const float
               reinterpret_cast<const float4*>(b.z), n, 1.f / float(n), b.dx, b.dw_part, b.loss};
     k_backward<<<unsigned((tiles + 3) / 4), kBlock, 0, s>>>(g);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    k_reduce<<<(kParams + 255) / 256, 256, 0, s>>>(b.dw_part, tiles, b.dw);
+    const uint64_t chunks = (tiles + kReduceChunk - 1) / kReduceChunk;
+    k_reduce_chunks<<<dim3((kParams + 255) / 256, unsigned(chunks)), 256, 0, s>>>(b.dw_part, tiles, b.dw_chunks);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    k_reduce<<<(kParams + 255) / 256, 256, 0, s>>>(b.dw_chunks, chunks, b.dw);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     k_grid_bwd<<<unsigned((n * kLevels + 255) / 256), 256, 0, s>>>(rec, box, b.dx, levels(), n,
                                                                     reinterpret_cast<float2*>(b.dtable));

@@ -142,6 +142,34 @@ def test_adam_step_matches_oracle_and_clears_gradients(device):
     assert float(dw2.abs().max()) == 0.0 and float(dtab2.abs().max()) == 0.0
 
 
+def test_flag0_records_do_not_poison_training(device):
+    """Records of paths that never reached the cache vertex hold whatever their buffer held;
+    training must ignore them: NaN/inf-filled flag-0 rows give the gradients of zero-filled
+    ones, and they are finite."""
+    w, table = _params(13)
+    rec = _records(2048, 4)
+    rec[::3, NO.R_FLAG] = 0.0
+    clean = rec.copy()
+    clean[::3, 1:] = 0.0
+    dirty = rec.copy()
+    dirty[::3, 1:] = np.nan
+    dirty[1::6, 1:] = np.where(dirty[1::6, NO.R_FLAG:NO.R_FLAG + 1] == 0.0, np.inf, dirty[1::6, 1:])
+    grads = []
+    for r in (clean, dirty):
+        c = RadianceCache(max_batch=2048, seed=3)
+        c.write_params(torch.from_numpy(w).to(device), torch.from_numpy(table).to(device))
+        c.backprop(torch.from_numpy(r).to(device), torch.from_numpy(BOX).to(device))
+        dw, dtab, loss = c.read_grads()
+        torch.cuda.synchronize()
+        grads.append((dw.cpu(), dtab.cpu(), loss.cpu()))
+    for k, (a, b) in enumerate(zip(grads[0], grads[1])):
+        assert bool(torch.isfinite(b).all())
+        if k == 0:  # weight gradients: per-tile products reduced in a fixed order
+            assert torch.equal(a, b)
+        else:  # table gradient and loss: float atomics, equal up to their order
+            torch.testing.assert_close(a, b, rtol=1e-5, atol=1e-9)
+
+
 def test_training_fits_a_smooth_radiance_field(device):
     c = RadianceCache(max_batch=8192, seed=2)
     box = torch.from_numpy(BOX).to(device)

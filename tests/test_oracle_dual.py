@@ -7,8 +7,11 @@ Perturbations are +-2^-14 along directions with entries in {-1, 0, 1}: exact in 
 float32 inputs.  A pixel whose path takes a different discrete decision at +h and -h (a
 hit accepted or not, an escape or shadow draw) has no derivative; those pixels are
 recognised by a finite difference far from both the dual derivative and from the smooth
-scale, and must be rare.  Everything else agrees to the O(h^2) error of the difference.
+scale (as are paths that cross a clamp, max(0, .) of an SH colour or a BRDF value), and
+must be rare (at most 2% of the pixels).  Everything else agrees to the O(h^2) error of the difference.
 """
+
+import zlib
 
 import numpy as np
 import pytest
@@ -60,7 +63,7 @@ def _direction(sc, block, rng):
 @pytest.mark.parametrize("block", ["position", "sigma", "rotation", "scale", "material", "sh", "snormal", "env"])
 def test_dual_derivative_matches_central_differences(max_bounces, block):
     sc, tab, opts = _scene(max_bounces)
-    rng = np.random.default_rng(hash((max_bounces, block)) % 2**31)
+    rng = np.random.default_rng(zlib.crc32(f"{max_bounces}/{block}".encode()))
     name, t = _direction(sc, block, rng)
     rays = O.Rays(sc.rays_ori, sc.rays_dir, sc.ray_to_world)
     base = {"density": sc.particle_density(), "material": sc.particle_material(), "sh": sc.features,
@@ -79,7 +82,8 @@ def test_dual_derivative_matches_central_differences(max_bounces, block):
         scale = np.abs(d).max() + 1e-12
         ok = np.abs(fd - d) <= 2e-4 * scale + 1e-3 * np.abs(d)
         bad = ~ok.reshape(ok.shape[0], ok.shape[1], ok.shape[2], -1).all(-1)
-        # a path whose discrete decisions differ between +h and -h jumps: O(1/h) differences
-        assert bad.mean() <= 0.01, (k, bad.sum())
+        # a path whose discrete decisions differ between +h and -h, or that crosses a clamp
+        # (max(0, .) of SH colours and BRDF values) between them, jumps: O(1/h) differences
+        assert bad.mean() <= 0.02, (k, bad.sum())
         touched += int((np.abs(d) > 1e-9 * scale).any(-1).sum()) if d.ndim > 3 else 0
     assert np.abs(der["pbr"]).max() > 0 or block in ("sh",)  # the direction moves the image
