@@ -216,6 +216,35 @@ ptir_status ptir_trace_bwd(ptir_tracer* tracer, const ptir_scene* scene, const p
                            const ptir_bwd_inputs* in, const ptir_bwd_outputs* out, ptir_counters* counters,
                            void* stream);
 
+/* Gradients of the forward outputs that ptir_trace_bwd ignores, for ptir_trace_bwd_exact.
+ * Each may be NULL (zero).  The normal, hit_distance2, distortion and hits_count outputs
+ * take no gradient. */
+typedef struct ptir_exact_grads {
+    const float* radiance_grad;        /* [..,3] primary radiance */
+    const float* density_grad;         /* [..,1] primary opacity */
+    const float* hit_distance_grad;    /* [..,2] channel 0 (integrated depth); channel 1 is ignored */
+    const float* shading_normal_grad;  /* [..,3] */
+    const float* pbr_components_grad;  /* [..,2,3] direct, indirect */
+} ptir_exact_grads;
+
+/* Exact-adjoint backward (no counterpart in the reference, whose backward leaves the density,
+ * SH and shading-normal gradients at zero and approximates the rest, referenceBwdOptix.cu:27-74):
+ * the gradient of sum_outputs <output_grad, output> with respect to every parameter of the
+ * trace -- particle_density [N,12] (position, density, quaternion, scale; the quaternion is
+ * differentiated as given, without normalisation), particle_material, particle_radiance,
+ * shading_normal and environment -- with the paths' discrete decisions (candidate sets,
+ * accepted hits, escape, shadow and roulette draws, sampled cells) held fixed.  Each path is
+ * replayed from its random stream, so the launch must match the forward's.  in->pbr_grad is
+ * required; light_grad and material_grad may be NULL; in->pbr, in->light and the tape are not
+ * read.  extra may be NULL.  cache_mode must be 0.  Allocates stream-ordered scratch of
+ * 112 * max(max_bounces, 1) bytes per path for the duration of the call.
+ * In this build the device launch is switched off: after argument checks the call returns
+ * PTIR_ERR_UNSUPPORTED (its kernel faulted the device and the cause is not found; the device
+ * code itself is verified on the host against the FP64 oracle's derivatives). */
+ptir_status ptir_trace_bwd_exact(ptir_tracer* tracer, const ptir_scene* scene, const ptir_launch* launch,
+                                 const ptir_bwd_inputs* in, const ptir_exact_grads* extra,
+                                 const ptir_bwd_outputs* out, void* stream);
+
 /* Bytes of a path tape for a [B,H,W] launch with max_bounces path vertices and room for
  * hits_per_path accepted hits per path (0 on invalid sizes).  No counterpart in the
  * reference: it lets ptir_trace_bwd skip re-tracing (a 64 B record per path and per

@@ -28,7 +28,7 @@ EXPORTED = ["ptir_create", "ptir_destroy", "ptir_build_bvh", "ptir_trace", "ptir
             "ptir_cache_query", "ptir_cache_backprop", "ptir_cache_step", "ptir_cache_train", "ptir_cache_last_error",
             "ptir_env_table_scratch_bytes", "ptir_env_table_build", "ptir_env_table_build_noalloc",
             "ptir_camera_rays", "ptir_resolve_forward", "ptir_resolve_backward", "ptir_adam_step",
-            "ptir_ragged_layout"]
+            "ptir_ragged_layout", "ptir_trace_bwd_exact"]
 
 
 class Options(C.Structure):
@@ -85,6 +85,11 @@ class BwdOutputs(C.Structure):
                                           "environment_grad")]
 
 
+class ExactGrads(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("radiance_grad", "density_grad", "hit_distance_grad",
+                                          "shading_normal_grad", "pbr_components_grad")]
+
+
 class Counters(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in ("paths", "segments", "candidates", "accepted", "accepted_primary",
                                           "shadow_candidates", "env_lookups", "mat_atomics", "env_atomics",
@@ -118,6 +123,8 @@ def lib() -> C.CDLL:
                                  C.POINTER(Counters), C.c_void_p]
         L.ptir_trace_bwd.argtypes = [C.c_void_p, C.POINTER(Scene), C.POINTER(Launch), C.POINTER(BwdInputs),
                                      C.POINTER(BwdOutputs), C.POINTER(Counters), C.c_void_p]
+        L.ptir_trace_bwd_exact.argtypes = [C.c_void_p, C.POINTER(Scene), C.POINTER(Launch), C.POINTER(BwdInputs),
+                                           C.POINTER(ExactGrads), C.POINTER(BwdOutputs), C.c_void_p]
         L.ptir_tape_bytes.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_uint32, C.c_uint32]
         L.ptir_tape_bytes.restype = C.c_uint64
         L.ptir_last_error.argtypes = [C.c_void_p]
@@ -147,7 +154,7 @@ def lib() -> C.CDLL:
         L.ptir_resolve_backward.argtypes = [C.POINTER(Resolve)] + [vp] * 10
         L.ptir_adam_step.argtypes = [C.POINTER(AdamGroup), i32, C.c_float, C.c_float, C.c_float, i32, vp, vp]
         L.ptir_ragged_layout.argtypes = [vp, i32, vp, vp, vp]
-        for name in ("ptir_create", "ptir_destroy", "ptir_build_bvh", "ptir_trace", "ptir_trace_bwd",
+        for name in ("ptir_create", "ptir_destroy", "ptir_build_bvh", "ptir_trace", "ptir_trace_bwd", "ptir_trace_bwd_exact",
                      "ptir_cache_create", "ptir_cache_destroy", "ptir_cache_read_params", "ptir_cache_write_params",
                      "ptir_cache_read_grads", "ptir_cache_query", "ptir_cache_backprop", "ptir_cache_step", "ptir_cache_train",
                      "ptir_env_table_build", "ptir_env_table_build_noalloc", "ptir_camera_rays", "ptir_resolve_forward",

@@ -193,6 +193,17 @@ struct BwdIO {
     int32_t replay_mode;  // 0: all paths, 1: only paths the tape kernel could not handle
 };
 
+// Exact adjoint: incoming gradients of the forward outputs (null = zero), the trace's particle
+// density, the parameter gradients (zeroed by the caller, original order) and the per-path
+// vertex records.
+struct AdjIO {
+    const float *pbr_grad, *light_grad, *mat_grad;                        // [..,3], [..,3], [..,5]
+    const float *rad_grad, *dns_grad, *hit_grad, *snrm_grad, *comp_grad;  // [..,3], [..,1], [..,2], [..,3], [..,6]
+    const float* density;                                                 // [N,12]
+    float *d_density, *d_material, *d_sh, *d_snormal, *d_env;
+    float4* scratch;
+};
+
 enum CounterId { C_PATHS = 0, C_SEGMENTS, C_CANDIDATES, C_ACCEPTED, C_ACCEPTED_PRIMARY, C_SHADOW_CANDIDATES,
                  C_ENV_LOOKUPS, C_MAT_ATOMICS, C_ENV_ATOMICS, C_TRAVERSALS, C_NODES, C_LEAF_TESTS,
                  C_COUNT };

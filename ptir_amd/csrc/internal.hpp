@@ -91,6 +91,9 @@ hipError_t resolve_backward(const ResolveIO& io, const ResolveGrads& g, hipStrea
 
 hipError_t launch_trace_fwd(const DeviceScene& S, const FwdOut& out, int kbuf, bool short_stack, hipStream_t s);
 hipError_t launch_trace_bwd(const DeviceScene& S, const BwdIO& io, int kbuf, bool short_stack, hipStream_t s);
+// Exact adjoint (adjoint.hip); io.scratch holds adjoint_scratch_bytes(paths, max_bounces) bytes.
+uint64_t adjoint_scratch_bytes(uint64_t paths, uint32_t max_bounces);
+hipError_t launch_trace_adjoint(const DeviceScene& S, const AdjIO& io, int kbuf, bool short_stack, hipStream_t s);
 
 constexpr int kTraceBlock = 128;  // 2 waves: a 16x8 pixel tile
 constexpr int kStackDeep = 64;    // traversal stack entries per lane (LDS): any accepted hierarchy

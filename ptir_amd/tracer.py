@@ -92,7 +92,7 @@ class HipTracer:
                  enable_normals: bool = True, enable_hitcounts: bool = True, enable_mis: bool = True,
                  enable_metallic: bool = False, visualize_lights: bool = False, kbuffer: int = 16,
                  use_tape: bool = True, tape_hits_per_path: int = 64, traversal: str = "wave",
-                 env_sampling: str = "alias"):
+                 env_sampling: str = "alias", gradients: str = "ptir"):
         if pipeline != "reference" or backward_pipeline not in ("referenceBwd", "reference" + "Bwd"):
             raise ValueError(f"unsupported pipeline pair ({pipeline}, {backward_pipeline}); "
                              "only the PTIR 'reference' path tracer is built")
@@ -103,6 +103,11 @@ class HipTracer:
         if env_sampling not in ("alias", "cdf"):
             raise ValueError(f"env_sampling must be 'alias' (caller-built alias table) or 'cdf' "
                              f"(envmap.build_cdf_table), not {env_sampling!r}")
+        if gradients not in ("ptir", "exact"):
+            raise ValueError(f"gradients must be 'ptir' (the reference's backward) or 'exact', not {gradients!r}")
+        # which backward the autograd function runs: the reference's path-replay backward
+        # (material and environment gradients) or the exact adjoint (every parameter)
+        self.gradients = gradients
         # layout of the environment table passed to trace / trace_bwd
         self.tab_kind = _lib.ENV_TABLE_ALIAS if env_sampling == "alias" else _lib.ENV_TABLE_CDF
         o = _lib.Options(float(particle_kernel_degree), float(particle_kernel_min_response),
@@ -342,6 +347,62 @@ class HipTracer:
         return d_dns, d_mat, d_sh, d_sn, d_env
 
 
+    # -------------------------------------------------------------- exact adjoint
+    def trace_bwd_exact(self, frame_id, ray_to_world, ray_ori, ray_dir, particle_density, particle_material,
+                        particle_radiance, particle_shading_normal, environment, environment_alias_table,
+                        ray_pbr_grd, ray_light_grd=None, ray_mat_grd=None, ray_rad_grd=None, ray_dns_grd=None,
+                        ray_hit_grd=None, ray_snrm_grd=None, ray_comp_grd=None, render_opts=0, sph_degree=3,
+                        min_transmittance=1e-3, max_bounces=3, ragged: RaggedBatch | None = None,
+                        rr_start: int = 0):
+        """Exact-adjoint backward (no counterpart in the reference; ptir_trace_bwd_exact): the
+        gradients (dDensity [N,12], dMaterial [N,5], dSH, dShadingNormal [N,3], dEnv [H,W,4])
+        of sum <output_grad, output> over the given output gradients, every one of them
+        propagated to every parameter with the paths' discrete decisions held fixed.
+        ray_hit_grd may hold one channel (integrated depth) or two (the second is ignored);
+        ray_comp_grd is [B,H,W,2,3] (direct, indirect).  The launch arguments must be the
+        forward's."""
+        sc, keep, env = self._scene(particle_density, particle_material, particle_radiance, particle_shading_normal,
+                                    environment, environment_alias_table, None, None, None)
+        la, keep2 = self._launch(frame_id, ray_to_world, ray_ori, ray_dir, render_opts, sph_degree,
+                                 min_transmittance, max_bounces, 0, ragged, rr_start)
+        if ragged is not None:
+            keep2.append(ragged.table)
+        B, H, W = la.B, la.H, la.W
+
+        def pix(t, name, c):
+            if t is None:
+                return None
+            t = _need(t, name)
+            if t.numel() != B * H * W * c:
+                raise ValueError(f"{name} must hold [{B},{H},{W},{c}] values, got {tuple(t.shape)}")
+            return t
+
+        if ray_hit_grd is not None and ray_hit_grd.numel() == B * H * W:
+            ray_hit_grd = torch.cat([ray_hit_grd.reshape(B, H, W, 1), torch.zeros_like(ray_hit_grd.reshape(B, H, W, 1))],
+                                    dim=-1)
+        if ray_pbr_grd is None:
+            raise ValueError("ray_pbr_grd is required")
+        grads = [pix(ray_pbr_grd, "ray_pbr_grd", 3), pix(ray_light_grd, "ray_light_grd", 3),
+                 pix(ray_mat_grd, "ray_mat_grd", 5), pix(ray_rad_grd, "ray_rad_grd", 3),
+                 pix(ray_dns_grd, "ray_dns_grd", 1), pix(ray_hit_grd, "ray_hit_grd", 2),
+                 pix(ray_snrm_grd, "ray_snrm_grd", 3), pix(ray_comp_grd, "ray_comp_grd", 6)]
+        n = sc.num_gaussians
+        f = dict(dtype=torch.float32, device=ray_dir.device)
+        d_dns = torch.empty(n, particle_density.shape[1], **f)
+        d_mat = torch.empty(n, particle_material.shape[1], **f)
+        d_sh = torch.empty(n, particle_radiance.shape[1], **f)
+        d_sn = torch.empty(n, particle_shading_normal.shape[1], **f)
+        d_env = torch.empty_like(environment) if env is not None else torch.zeros(0, 0, 4, **f)
+        bi = _lib.BwdInputs(None, None, _ptr(grads[0]), _ptr(grads[1]), _ptr(grads[2]))
+        ex = _lib.ExactGrads(*[_ptr(g) for g in grads[3:]])
+        bo = _lib.BwdOutputs(_ptr(d_dns), _ptr(d_mat), _ptr(d_sh), _ptr(d_sn), _ptr(d_env) if env is not None else None)
+        st = _lib.lib().ptir_trace_bwd_exact(self._h, C.byref(sc), C.byref(la), C.byref(bi), C.byref(ex),
+                                             C.byref(bo), _stream())
+        _lib.check(st, self._h)
+        del keep, keep2, grads
+        return d_dns, d_mat, d_sh, d_sn, d_env
+
+
 # ----------------------------------------------------------------------------- light buffers
 @dataclass(frozen=True)
 class LightBuffers:
@@ -417,7 +478,8 @@ class Tracer:
             light_buffers = light_buffers.to(ray_dir.device)
             lights, lat, mlv, mlt, ml, mlta = light_buffers.tensors()
             tape = None
-            if getattr(tracer_wrapper, "use_tape", False) and any(ctx.needs_input_grad):
+            exact = getattr(tracer_wrapper, "gradients", "ptir") == "exact"
+            if not exact and getattr(tracer_wrapper, "use_tape", False) and any(ctx.needs_input_grad):
                 tape = tracer_wrapper.new_tape(ray_dir, max_bounces)
             (ray_radiance, ray_density, ray_hit_distance, ray_hit_distance_second_moment, ray_depth_distortion,
              ray_normals, ray_shadingnormal, ray_material, hits_count, mog_visibility, ray_pbr, ray_light,
@@ -446,7 +508,7 @@ class Tracer:
         def backward(ctx, ray_radiance_grd, ray_density_grd, ray_hit_distance_grd, ray_hit_distance_second_moment_grd,
                      ray_depth_distortion_grd, ray_normals_grd, ray_shadingnormal_grd, ray_material_grd,
                      ray_hits_count_grd_UNUSED, mog_visibility_grd_UNUSED, ray_pbr_grd, ray_light_grd,
-                     pbr_components_grd_UNUSED):
+                     pbr_components_grd):
             (ray_to_world, ray_ori, ray_dir, ray_radiance, ray_density, ray_hit_distance,
              ray_hit_distance_second_moment, ray_depth_distortion, ray_normals, ray_shadingnormal, ray_material,
              ray_pbr, ray_light, particle_density, particle_material, mog_sph, mog_snrm, environment,
@@ -457,6 +519,17 @@ class Tracer:
                 ray_pbr_grd = torch.zeros_like(ray_pbr)
             if ray_material_grd is not None:
                 ray_material_grd = ray_material_grd.contiguous()
+            if getattr(ctx.tracer_wrapper, "gradients", "ptir") == "exact":
+                c = lambda t: None if t is None else t.contiguous()  # noqa: E731
+                (particle_density_grd, particle_material_grd, mog_sph_grd, mog_sn_grd,
+                 environment_grd) = ctx.tracer_wrapper.trace_bwd_exact(
+                    ctx.frame_id, ray_to_world, ray_ori, ray_dir, particle_density, particle_material, mog_sph,
+                    mog_snrm, environment, environment_alias_table, ray_pbr_grd.contiguous(),
+                    ray_light_grd.contiguous(), ray_material_grd, c(ray_radiance_grd), c(ray_density_grd),
+                    c(ray_hit_distance_grd), c(ray_shadingnormal_grd), c(pbr_components_grd),
+                    ctx.render_opts, ctx.sph_degree, ctx.min_transmittance, ctx.max_bounces)
+                return Tracer._Autograd._split(particle_density_grd, particle_material_grd, mog_sph_grd, mog_sn_grd,
+                                               environment_grd)
             (particle_density_grd, particle_material_grd, mog_sph_grd, mog_sn_grd,
              environment_grd) = ctx.tracer_wrapper.trace_bwd(
                 ctx.frame_id, ray_to_world, ray_ori, ray_dir, ray_radiance, ray_density, ray_hit_distance,
@@ -468,6 +541,11 @@ class Tracer:
                 ctx.render_opts, ctx.sph_degree, ctx.min_transmittance, ctx.max_bounces,
                 **({"tape": ctx.tape} if ctx.tape is not None else {}))
             ctx.tape = None
+            return Tracer._Autograd._split(particle_density_grd, particle_material_grd, mog_sph_grd, mog_sn_grd,
+                                           environment_grd)
+
+        @staticmethod
+        def _split(particle_density_grd, particle_material_grd, mog_sph_grd, mog_sn_grd, environment_grd):
             mog_pos_grd, mog_dns_grd, mog_rot_grd, mog_scl_grd, _ = torch.split(particle_density_grd, [3, 1, 4, 3, 1],
                                                                                 dim=1)
             mog_malb_grd, mog_mrgh_grd, mog_mmet_grd = torch.split(particle_material_grd, [3, 1, 1], dim=1)
