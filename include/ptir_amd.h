@@ -49,6 +49,14 @@ typedef struct ptir_options {
     int32_t wave_traversal;       /* 0: every ray walks the hierarchy on its own; 1: the primary rays
                                      of an 8x8 pixel tile walk it together (the same hits in the
                                      same order, faster; secondary rays always walk on their own) */
+    int32_t schedule;             /* forward schedule (no counterpart in the reference): 0 = one lane
+                                     follows its path from start to end (the reference's raygen);
+                                     1 = wavefront: the primary segments, the NEE of the paths that
+                                     reached an interaction and each further bounce run as separate
+                                     launches over compacted queues of live paths, with the same
+                                     per-path decisions and outputs.  The tracer then keeps a device
+                                     workspace of 164 bytes per path, grown on demand (by a host-
+                                     synchronous allocation: make the first call outside graph capture) */
 } ptir_options;
 
 /* Scene inputs of one trace call (OptixTracer::trace argument list,

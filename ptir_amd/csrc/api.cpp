@@ -16,6 +16,8 @@ struct ptir_tracer {
     ptir::BvhBuffers bvh;
     bool built = false;
     bool short_stack = false;  // the built hierarchy fits the short traversal stack
+    void* wf = nullptr;        // wavefront-schedule workspace
+    uint64_t wf_bytes = 0;
     std::string err;
 };
 
@@ -202,6 +204,8 @@ ptir_status ptir_create(const ptir_options* options, ptir_tracer** out) {
         return fail(nullptr, PTIR_ERR_INVALID_ARGUMENT, "kbuffer must be 8, 16 or 32");
     if (options->wave_traversal != 0 && options->wave_traversal != 1)
         return fail(nullptr, PTIR_ERR_INVALID_ARGUMENT, "wave_traversal must be 0 or 1");
+    if (options->schedule != 0 && options->schedule != 1)
+        return fail(nullptr, PTIR_ERR_INVALID_ARGUMENT, "schedule must be 0 (one lane per path) or 1 (wavefront)");
     ptir_tracer* t = new (std::nothrow) ptir_tracer();
     if (!t) return fail(nullptr, PTIR_ERR_OUT_OF_MEMORY, "host allocation failed");
     t->opts = *options;
@@ -212,6 +216,7 @@ ptir_status ptir_create(const ptir_options* options, ptir_tracer** out) {
 ptir_status ptir_destroy(ptir_tracer* t) {
     if (!t) return PTIR_OK;
     ptir::bvh_free(t->bvh);
+    if (t->wf) (void)hipFree(t->wf);
     delete t;
     return PTIR_OK;
 }
@@ -277,7 +282,28 @@ ptir_status ptir_trace(ptir_tracer* t, const ptir_scene* sc, const ptir_launch* 
         e = ptir::fill_zero(o.tape.base + ptir::kTapePoolNext, sizeof(uint32_t), s);
         if (e != hipSuccess) return hip_fail(t, e, "zeroing the tape pool counter");
     }
-    e = ptir::launch_trace_fwd(S, o, t->opts.kbuffer, t->short_stack, s);
+    if (t->opts.schedule == 1) {
+        const uint64_t need = ptir::wavefront_bytes(uint64_t(S.B) * S.H * S.W, S.max_bounces);
+        if (t->wf_bytes < need) {
+            // grow the workspace; the old one may still be in use by queued work
+            if (t->wf) {
+                e = hipDeviceSynchronize();
+                if (e == hipSuccess) e = hipFree(t->wf);
+                t->wf = nullptr;
+                t->wf_bytes = 0;
+                if (e != hipSuccess) return hip_fail(t, e, "releasing the wavefront workspace");
+            }
+            e = hipMalloc(&t->wf, need);
+            if (e != hipSuccess) {
+                t->wf = nullptr;
+                return fail(t, PTIR_ERR_OUT_OF_MEMORY, std::string("wavefront workspace: ") + hipGetErrorString(e));
+            }
+            t->wf_bytes = need;
+        }
+        e = ptir::launch_trace_fwd_wavefront(S, o, t->wf, t->opts.kbuffer, t->short_stack, s);
+    } else {
+        e = ptir::launch_trace_fwd(S, o, t->opts.kbuffer, t->short_stack, s);
+    }
     if (e != hipSuccess) return hip_fail(t, e, "trace kernel launch");
     return end_counters(t, counters, s);
 }

@@ -91,6 +91,10 @@ hipError_t resolve_backward(const ResolveIO& io, const ResolveGrads& g, hipStrea
 
 hipError_t launch_trace_fwd(const DeviceScene& S, const FwdOut& out, int kbuf, bool short_stack, hipStream_t s);
 hipError_t launch_trace_bwd(const DeviceScene& S, const BwdIO& io, int kbuf, bool short_stack, hipStream_t s);
+// Wavefront schedule of the forward (wavefront.hip); `work` holds wavefront_bytes(paths, max_bounces).
+uint64_t wavefront_bytes(uint64_t paths, uint32_t max_bounces);
+hipError_t launch_trace_fwd_wavefront(const DeviceScene& S, const FwdOut& out, void* work, int kbuf, bool short_stack,
+                                      hipStream_t s);
 // Exact adjoint (adjoint.hip); io.scratch holds adjoint_scratch_bytes(paths, max_bounces) bytes.
 uint64_t adjoint_scratch_bytes(uint64_t paths, uint32_t max_bounces);
 hipError_t launch_trace_adjoint(const DeviceScene& S, const AdjIO& io, int kbuf, bool short_stack, hipStream_t s);

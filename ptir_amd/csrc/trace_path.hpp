@@ -1361,6 +1361,105 @@ __device__ __forceinline__ void path_init(Path& p) {
 }
 
 
+// referenceOptix.cu:27-71: one path from its primary ray to its end (the body of k_trace_fwd).
+// wstack: the wave's node stack for the wave-coherent primary traversal, or null.
+template <int K, int SD>
+__device__ __forceinline__ void fwd_path(const DeviceScene& S, const FwdOut& out, const Lane& L, uint32_t* wstack,
+                                         const PixelId& px) {
+    const size_t pix = px.pix;
+    const size_t paths = size_t(S.B) * S.H * S.W;
+    const bool rec = out.tape.base != nullptr;
+    uint32_t* prec = rec ? out.tape.path(pix) : nullptr;
+    if (rec && pix == 0) {
+        uint32_t* h = out.tape.base;
+        h[0] = kTapeMagic; h[1] = kTapeVersion; h[2] = uint32_t(S.B); h[3] = uint32_t(S.H); h[4] = uint32_t(S.W);
+        h[5] = out.tape.segs; h[6] = S.n; h[7] = S.frame; h[8] = out.tape.cap; h[9] = tape_pool_chunks(paths);
+    }
+    uint32_t flags = 0u;
+    SegTape tp;
+    tp.hits = nullptr;
+    float* crec = out.cache ? out.cache + size_t(nrc::kRecFloats) * pix : nullptr;
+    if (crec) crec[nrc::kRecFlag] = 0.f;
+    Rng rng;
+    rng.seed(Rng::launch_seed(uint32_t(px.x), uint32_t(px.y), uint32_t(px.z), S.frame));
+    Path p;
+    path_init(p);
+    count(S, C_PATHS);
+    uint32_t used = 0u;  // tape hit entries used by this path
+    if (rec) {
+        path_tape_begin(tp, out.tape, pix, paths);
+        seg_tape_begin(tp, used);
+    }
+    ray_intersect<K, SD>(S, primary_ray(S, pix), p.cur, rng, false, true, true, out.vis, L, tp,
+                     S.wave_traversal ? wstack : nullptr);
+    if (rec) {
+        store_seg(prec, 0, p.cur, tp);
+        flags |= tp.overflow ? kTapeOverflow : 0u;
+        used += tp.n;
+    }
+    {
+        const Seg& c = p.cur;
+        float* o;
+        o = out.rad + 3 * pix; o[0] = c.radiance.x; o[1] = c.radiance.y; o[2] = c.radiance.z;
+        out.dns[pix] = 1.f - c.T;
+        o = out.hit + 2 * pix; o[0] = c.hitDist; o[1] = c.lastHit;
+        out.hit2[pix] = c.hitDist2;
+        out.dist[pix] = c.distortion;
+        o = out.nrm + 3 * pix; o[0] = c.normal.x; o[1] = c.normal.y; o[2] = c.normal.z;
+        o = out.snrm + 3 * pix; o[0] = c.sn.x; o[1] = c.sn.y; o[2] = c.sn.z;
+        o = out.mat + 5 * pix; o[0] = c.alb.x; o[1] = c.alb.y; o[2] = c.alb.z; o[3] = c.rough; o[4] = c.metal;
+        out.hits[pix] = c.hits;
+    }
+    if (p.cur.ivalid) {
+        if (S.enable_mis) sample_nee<K, SD>(S, p, rng, L);
+        if (rec) {
+            prec[1] = uint32_t(rng.state);
+            prec[2] = uint32_t(rng.state >> 32);
+            float4* e = reinterpret_cast<float4*>(prec + 4);
+            e[0] = make_float4(p.eray.d.x, p.eray.d.y, p.eray.d.z, p.elightPdf);
+            e[1] = make_float4(p.escatterPdf, p.econtrib.x, p.econtrib.y, p.econtrib.z);
+        }
+        for (unsigned depth = 0; depth < S.max_bounces && p.active; ++depth) {
+            accumulate(S, p);
+            p.active &= (depth + 1u < S.max_bounces) && p.cur.ivalid;
+            if (!p.active) break;
+            p.numBounces = depth + 1u;
+            if (crec && p.numBounces == 2u) {
+                cache_record(S, p, crec);
+                if (S.cache_mode == 1u) break;  // the cache stands in for the rest of the path
+            }
+            sample_next(S, p, rng);
+            if (vmax(p.thr) < 1e-4f) break;
+            if (!russian_roulette(S, p, rng)) break;
+            if (rec) seg_tape_begin(tp, used);
+            ray_intersect<K, SD>(S, p.cur.ray, p.cur, rng, true, false, true, out.vis, L, tp);
+            if (rec) {
+                store_seg(prec, p.numBounces, p.cur, tp);
+                flags |= tp.overflow ? kTapeOverflow : 0u;
+                used += tp.n;
+            }
+        }
+    }
+    if (rec) {
+        prec[0] = flags;
+        prec[3] = tp.chunk;
+    }
+    if (crec && S.cache_mode == 2u && crec[nrc::kRecFlag] != 0.f) {
+        // training target: the radiance the rest of the path carried, per unit throughput at the vertex
+        float* tg = crec + nrc::kRecTarget;
+        const float* th = crec + nrc::kRecThr;
+        tg[0] = th[0] > 0.f ? (p.L.x - tg[0]) / th[0] : 0.f;
+        tg[1] = th[1] > 0.f ? (p.L.y - tg[1]) / th[1] : 0.f;
+        tg[2] = th[2] > 0.f ? (p.L.z - tg[2]) / th[2] : 0.f;
+    }
+    float* o;
+    o = out.pbr + 3 * pix; o[0] = p.L.x; o[1] = p.L.y; o[2] = p.L.z;
+    o = out.light + 3 * pix; o[0] = p.Lnb.x; o[1] = p.Lnb.y; o[2] = p.Lnb.z;
+    o = out.comp + 6 * pix;
+    o[0] = p.direct.x; o[1] = p.direct.y; o[2] = p.direct.z;
+    o[3] = p.indirect.x; o[4] = p.indirect.y; o[5] = p.indirect.z;
+}
+
 static inline void tiling(DeviceScene& S) {
     if (S.num_images > 0) return;  // ragged: num_blocks is the table's tile count
     S.tiles_x = (S.W + 15) / 16;

@@ -92,7 +92,7 @@ class HipTracer:
                  enable_normals: bool = True, enable_hitcounts: bool = True, enable_mis: bool = True,
                  enable_metallic: bool = False, visualize_lights: bool = False, kbuffer: int = 16,
                  use_tape: bool = True, tape_hits_per_path: int = 64, traversal: str = "wave",
-                 env_sampling: str = "alias", gradients: str = "ptir"):
+                 env_sampling: str = "alias", gradients: str = "ptir", schedule: str = "megakernel"):
         if pipeline != "reference" or backward_pipeline not in ("referenceBwd", "reference" + "Bwd"):
             raise ValueError(f"unsupported pipeline pair ({pipeline}, {backward_pipeline}); "
                              "only the PTIR 'reference' path tracer is built")
@@ -103,6 +103,8 @@ class HipTracer:
         if env_sampling not in ("alias", "cdf"):
             raise ValueError(f"env_sampling must be 'alias' (caller-built alias table) or 'cdf' "
                              f"(envmap.build_cdf_table), not {env_sampling!r}")
+        if schedule not in ("megakernel", "wavefront"):
+            raise ValueError(f"schedule must be 'megakernel' (one lane per path) or 'wavefront', not {schedule!r}")
         if gradients not in ("ptir", "exact"):
             raise ValueError(f"gradients must be 'ptir' (the reference's backward) or 'exact', not {gradients!r}")
         # which backward the autograd function runs: the reference's path-replay backward
@@ -114,7 +116,7 @@ class HipTracer:
                          int(bool(particle_kernel_density_clamping)), int(particle_radiance_sph_degree),
                          int(bool(enable_normals)), int(bool(enable_hitcounts)), int(bool(enable_mis)),
                          int(bool(enable_metallic)), int(bool(visualize_lights)), int(kbuffer),
-                         {"lane": 0, "wave": 1}[traversal])
+                         {"lane": 0, "wave": 1}[traversal], {"megakernel": 0, "wavefront": 1}[schedule])
         h = C.c_void_p()
         L = _lib.lib()
         _lib.check(L.ptir_create(C.byref(o), C.byref(h)), None)

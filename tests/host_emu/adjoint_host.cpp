@@ -1,260 +1,40 @@
-// Test harness: runs the exact-adjoint device code (ptir_amd/csrc/adjoint_path.hpp, with
-// the traversal and path code of trace_path.hpp) on the CPU, one lane at a time, over a
-// launch described by the files of one directory (written by tests/test_exact_adjoint_host.py):
-//   meta.txt            "key value" lines (sizes and launch options)
-//   <name>.f32          float32 inputs: density [N,12], material [N,5], sh [N,S], snormal [N,3],
-//                       env [h,w,4], table [3,th,tw], ori/dir [B,H,W,3], ray_to_world [12],
-//                       output gradients g_pbr, g_light, g_mat, g_rad, g_dns, g_hit, g_snrm, g_comp
-// and writes d_density, d_material, d_sh, d_snormal, d_env (.f32).
-//
-// The hierarchy is a median split with one Gaussian per leaf and the per-leaf records are
-// filled as k_records fills them (bvh.hip); the traversal returns the same hits for any
-// hierarchy, so the gradients are those of the device kernel on the same inputs.
-#include <cstdio>
-#include <cstring>
-#include <functional>
-#include <vector>
-
+// Test harness: runs the exact-adjoint device code (ptir_amd/csrc/adjoint_path.hpp, with the
+// traversal and path code of trace_path.hpp) on the CPU, one lane at a time, over the launch
+// of host_scene.hpp plus the output gradients g_pbr, g_light, g_mat, g_rad, g_dns, g_hit,
+// g_snrm, g_comp (.f32), and writes d_density, d_material, d_sh, d_snormal, d_env (.f32).
 #include "../../ptir_amd/csrc/adjoint_path.hpp"
+#include "host_scene.hpp"
 
-Dim3 threadIdx, blockIdx, blockDim;
 using namespace ptir;
-
-namespace {
-
-const char* g_dir;
-
-std::vector<float> read_f32(const char* name, size_t n) {
-    std::vector<float> v(n, 0.f);
-    char path[1024];
-    snprintf(path, sizeof(path), "%s/%s.f32", g_dir, name);
-    FILE* f = fopen(path, "rb");
-    if (!f) {
-        fprintf(stderr, "missing %s\n", path);
-        exit(2);
-    }
-    const size_t got = fread(v.data(), 4, n, f);
-    fclose(f);
-    if (got != n) {
-        fprintf(stderr, "%s: %zu of %zu floats\n", path, got, n);
-        exit(2);
-    }
-    return v;
-}
-
-void write_f32(const char* name, const std::vector<float>& v) {
-    char path[1024];
-    snprintf(path, sizeof(path), "%s/%s.f32", g_dir, name);
-    FILE* f = fopen(path, "wb");
-    fwrite(v.data(), 4, v.size(), f);
-    fclose(f);
-}
-
-struct Meta {
-    char key[64][64];
-    double val[64];
-    int n = 0;
-    double operator[](const char* k) const {
-        for (int i = 0; i < n; ++i)
-            if (!strcmp(key[i], k)) return val[i];
-        fprintf(stderr, "meta.txt has no %s\n", k);
-        exit(2);
-    }
-};
-
-struct Box {
-    float lo[3], hi[3];
-};
-
-// kernelScale (particlePrimitives.cu:27-51), as bvh.hip computes it
-float kernel_scale(float density, int degree, float min_response, int clamping) {
-    const float modulation = clamping ? density : 1.0f;
-    const float minR = fminf(min_response / modulation, 0.97f);
-    if (degree == 0) return ((1.0f - minR) / 3.0f) / -0.329630334487f;
-    const float a = -4.5f / powf(3.0f, float(degree));
-    return powf(logf(minR) / a, 1.0f / float(degree));
-}
-
-void rotation(const float* q, float R[3][3]) {
-    const float w = q[0], x = q[1], y = q[2], z = q[3];
-    R[0][0] = 1 - 2 * (y * y + z * z); R[0][1] = 2 * (x * y - w * z); R[0][2] = 2 * (x * z + w * y);
-    R[1][0] = 2 * (x * y + w * z); R[1][1] = 1 - 2 * (x * x + z * z); R[1][2] = 2 * (y * z - w * x);
-    R[2][0] = 2 * (x * z - w * y); R[2][1] = 2 * (y * z + w * x); R[2][2] = 1 - 2 * (x * x + y * y);
-}
-
-}  // namespace
 
 int main(int argc, char** argv) {
     if (argc != 2) {
         fprintf(stderr, "usage: adjoint_host <dir>\n");
         return 2;
     }
-    g_dir = argv[1];
-    Meta m;
-    {
-        char path[1024];
-        snprintf(path, sizeof(path), "%s/meta.txt", g_dir);
-        FILE* f = fopen(path, "r");
-        if (!f) return 2;
-        while (m.n < 64 && fscanf(f, "%63s %lf", m.key[m.n], &m.val[m.n]) == 2) ++m.n;
-        fclose(f);
-    }
-    const int N = int(m["N"]), B = int(m["B"]), H = int(m["H"]), W = int(m["W"]);
-    const int eh = int(m["env_h"]), ew = int(m["env_w"]), th = int(m["tab_h"]), tw = int(m["tab_w"]);
-    const int shs = int(m["sh_stride"]);
-    const int degree = int(m["kernel_degree"]), clamping = int(m["density_clamping"]);
-    const float min_response = float(m["min_response"]);
-    const size_t P = size_t(B) * H * W;
-    const auto dens = read_f32("density", size_t(N) * 12), mat = read_f32("material", size_t(N) * 5);
-    const auto sh = read_f32("sh", size_t(N) * shs), sn = read_f32("snormal", size_t(N) * 3);
-    const auto env = read_f32("env", size_t(eh) * ew * 4), tab = read_f32("table", size_t(3) * th * tw);
-    const auto ori = read_f32("ori", P * 3), dir = read_f32("dir", P * 3), T12 = read_f32("ray_to_world", 12);
-    const auto g_pbr = read_f32("g_pbr", P * 3), g_light = read_f32("g_light", P * 3), g_mat = read_f32("g_mat", P * 5);
-    const auto g_rad = read_f32("g_rad", P * 3), g_dns = read_f32("g_dns", P), g_hit = read_f32("g_hit", P * 2);
-    const auto g_snrm = read_f32("g_snrm", P * 3), g_comp = read_f32("g_comp", P * 6);
-
-    // instance boxes (k_instances), then a median split with one Gaussian per leaf
-    std::vector<Box> box(N);
-    for (int i = 0; i < N; ++i) {
-        const float* p = &dens[12 * size_t(i)];
-        float R[3][3];
-        rotation(p + 4, R);
-        const float ks = kernel_scale(p[3], degree, min_response, clamping);
-        for (int r = 0; r < 3; ++r) {
-            const float ext = fabsf(R[r][0]) * ks * p[8] + fabsf(R[r][1]) * ks * p[9] + fabsf(R[r][2]) * ks * p[10];
-            const float pad = 2e-6f * (fabsf(p[r]) + ext) + 1e-7f;
-            box[i].lo[r] = p[r] - ext - pad;
-            box[i].hi[r] = p[r] + ext + pad;
-        }
-    }
-    std::vector<int> order(N), leaf_of;
-    for (int i = 0; i < N; ++i) order[i] = i;
-    std::vector<float4> nodes;
-    std::function<int(int, int, Box&)> build = [&](int a, int b, Box& out) -> int {
-        if (b - a == 1) {
-            leaf_of.push_back(order[a]);
-            out = box[order[a]];
-            return ~int(leaf_of.size() - 1);
-        }
-        float clo[3] = {1e30f, 1e30f, 1e30f}, chi[3] = {-1e30f, -1e30f, -1e30f};
-        for (int k = a; k < b; ++k)
-            for (int r = 0; r < 3; ++r) {
-                clo[r] = fminf(clo[r], dens[12 * size_t(order[k]) + r]);
-                chi[r] = fmaxf(chi[r], dens[12 * size_t(order[k]) + r]);
-            }
-        int ax = 0;
-        for (int r = 1; r < 3; ++r)
-            if (chi[r] - clo[r] > chi[ax] - clo[ax]) ax = r;
-        const int mid = (a + b) / 2;
-        std::nth_element(order.begin() + a, order.begin() + mid, order.begin() + b, [&](int i, int j) {
-            const float ki = dens[12 * size_t(i) + ax], kj = dens[12 * size_t(j) + ax];
-            return ki < kj || (ki == kj && i < j);
-        });
-        const int me = int(nodes.size() / 4);
-        nodes.resize(nodes.size() + 4);
-        Box bl, br;
-        const int l = build(a, mid, bl);
-        const int r = build(mid, b, br);
-        float4* nd = &nodes[4 * size_t(me)];
-        nd[0] = make_float4(bl.lo[0], bl.lo[1], bl.lo[2], __int_as_float(l));
-        nd[1] = make_float4(bl.hi[0], bl.hi[1], bl.hi[2], __int_as_float(r));
-        nd[2] = make_float4(br.lo[0], br.lo[1], br.lo[2], 0.f);
-        nd[3] = make_float4(br.hi[0], br.hi[1], br.hi[2], 0.f);
-        for (int q = 0; q < 3; ++q) {
-            out.lo[q] = fminf(bl.lo[q], br.lo[q]);
-            out.hi[q] = fmaxf(bl.hi[q], br.hi[q]);
-        }
-        return me;
-    };
-    Box all;
-    const int root = build(0, N, all);
-    std::vector<float> bounds = {all.lo[0], all.lo[1], all.lo[2], all.hi[0], all.hi[1], all.hi[2]};
-    // per-leaf records (k_records)
-    std::vector<float4> grec(4 * size_t(N)), srec(2 * size_t(N)), prec(N);
-    for (int j = 0; j < N; ++j) {
-        const int i = leaf_of[j];
-        const float* p = &dens[12 * size_t(i)];
-        float R[3][3];
-        rotation(p + 4, R);
-        const float is[3] = {1.0f / p[8], 1.0f / p[9], 1.0f / p[10]};
-        grec[4 * j + 0] = make_float4(R[0][0] * is[0], R[1][0] * is[0], R[2][0] * is[0], p[0]);
-        grec[4 * j + 1] = make_float4(R[0][1] * is[1], R[1][1] * is[1], R[2][1] * is[1], p[1]);
-        grec[4 * j + 2] = make_float4(R[0][2] * is[2], R[1][2] * is[2], R[2][2] * is[2], p[2]);
-        grec[4 * j + 3] = make_float4(p[3], kernel_scale(p[3], degree, min_response, clamping), __int_as_float(i), 0.f);
-        const float* mt = &mat[5 * size_t(i)];
-        srec[2 * j] = make_float4(mt[0], mt[1], mt[2], mt[3]);
-        srec[2 * j + 1] = make_float4(mt[4], sn[3 * size_t(i)], sn[3 * size_t(i) + 1], sn[3 * size_t(i) + 2]);
-        prec[j] = make_float4(R[0][0] * p[8] + R[0][1] * p[9] + R[0][2] * p[10],
-                              R[1][0] * p[8] + R[1][1] * p[9] + R[1][2] * p[10],
-                              R[2][0] * p[8] + R[2][1] * p[9] + R[2][2] * p[10], 1.f);
-    }
-    // the launch, as make_scene (api.cpp) fills it
-    int error_flag = 0;
-    DeviceScene S{};
-    S.grec = grec.data();
-    S.srec = srec.data();
-    S.prec = prec.data();
-    S.nodes = nodes.data();
-    S.bounds = bounds.data();
-    S.sh = sh.data();
-    S.env = reinterpret_cast<const float4*>(env.data());
-    S.tab = tab.data();
-    S.ray_to_world = T12.data();
-    S.ray_ori = ori.data();
-    S.ray_dir = dir.data();
-    S.n = uint32_t(N);
-    S.root = N > 1 ? root : ~0;
-    S.sh_stride = shs;
-    S.sph_degree = int(m["sph_degree"]);
-    S.env_w = ew;
-    S.env_h = eh;
-    S.tab_w = tw;
-    S.tab_h = th;
-    S.tab_kind = int(m["tab_kind"]);
-    S.kernel_degree = degree;
-    S.min_response = min_response;
-    S.min_transmittance = float(m["min_transmittance"]);
-    S.render_opts = unsigned(m["render_opts"]);
-    const unsigned mb = unsigned(m["max_bounces"]);
-    S.max_bounces = S.render_opts ? (mb < 2u ? mb : 2u) : mb;
-    S.frame = unsigned(m["frame"]);
-    S.cache_mode = 0;
-    S.B = B;
-    S.H = H;
-    S.W = W;
-    S.enable_normals = int(m["enable_normals"]);
-    S.enable_hitcounts = 1;
-    S.enable_mis = int(m["enable_mis"]);
-    S.enable_metallic = int(m["enable_metallic"]);
-    S.has_snormal = 1;
-    S.wave_traversal = 0;
-    S.images = nullptr;
-    S.num_images = 0;
-    S.rr_start = unsigned(m["rr_start"]);
-    S.counters = nullptr;
-    S.error_flag = &error_flag;
-    tiling(S);
-    std::vector<float> d_density(size_t(N) * 12, 0.f), d_material(size_t(N) * 5, 0.f), d_sh(size_t(N) * shs, 0.f),
-        d_snormal(size_t(N) * 3, 0.f), d_env(size_t(eh) * ew * 4, 0.f);
-    std::vector<float4> scratch(adjoint_scratch_words(P, S.max_bounces));
+    host::Launch la(argv[1]);
+    const size_t P = la.P;
+    const int N = la.N;
+    const auto g_pbr = host::read_f32("g_pbr", P * 3), g_light = host::read_f32("g_light", P * 3);
+    const auto g_mat = host::read_f32("g_mat", P * 5), g_rad = host::read_f32("g_rad", P * 3);
+    const auto g_dns = host::read_f32("g_dns", P), g_hit = host::read_f32("g_hit", P * 2);
+    const auto g_snrm = host::read_f32("g_snrm", P * 3), g_comp = host::read_f32("g_comp", P * 6);
+    std::vector<float> d_density(size_t(N) * 12, 0.f), d_material(size_t(N) * 5, 0.f),
+        d_sh(size_t(N) * la.shs, 0.f), d_snormal(size_t(N) * 3, 0.f), d_env(size_t(la.eh) * la.ew * 4, 0.f);
+    std::vector<float4> scratch(adjoint_scratch_words(P, la.S.max_bounces));
     AdjIO io{g_pbr.data(), g_light.data(), g_mat.data(), g_rad.data(), g_dns.data(), g_hit.data(), g_snrm.data(),
-             g_comp.data(), dens.data(), d_density.data(), d_material.data(), d_sh.data(), d_snormal.data(),
+             g_comp.data(), la.dens.data(), d_density.data(), d_material.data(), d_sh.data(), d_snormal.data(),
              d_env.data(), scratch.data()};
     std::vector<uint32_t> lds(size_t(LdsLayout<16, kStackDeep>::kWords) * kTraceBlock, 0u);
-    for (int b = 0; b < S.num_blocks; ++b) {
-        for (int t = 0; t < kTraceBlock; ++t) {
-            blockIdx.x = unsigned(b);
-            threadIdx.x = unsigned(t);
-            const Lane L{lds.data() + t};
-            const PixelId px = pixel_of(S);
-            if (px.ok) trace_adjoint_path<16, kStackDeep>(S, io, L, px);
-        }
-    }
-    write_f32("d_density", d_density);
-    write_f32("d_material", d_material);
-    write_f32("d_sh", d_sh);
-    write_f32("d_snormal", d_snormal);
-    write_f32("d_env", d_env);
-    printf("%d blocks, traversal stack overflow flag %d\n", S.num_blocks, error_flag);
-    return error_flag;
+    la.for_each_path([&](const PixelId& px, int t) {
+        const Lane L{lds.data() + t};
+        trace_adjoint_path<16, kStackDeep>(la.S, io, L, px);
+    });
+    host::write_f32("d_density", d_density);
+    host::write_f32("d_material", d_material);
+    host::write_f32("d_sh", d_sh);
+    host::write_f32("d_snormal", d_snormal);
+    host::write_f32("d_env", d_env);
+    printf("%d blocks, traversal stack overflow flag %d\n", la.S.num_blocks, la.error_flag);
+    return la.error_flag;
 }

@@ -33,7 +33,7 @@ def test_library_loads_and_exports_every_declared_symbol():
 def _opts(**kw):
     o = dict(kernel_degree=4.0, min_response=0.0113, density_clamping=1, sph_max_degree=3, enable_normals=1,
              enable_hitcounts=1, enable_mis=1, enable_metallic=0, visualize_lights=0, kbuffer=16,
-             wave_traversal=1)
+             wave_traversal=1, schedule=1)
     o.update(kw)
     return _lib.Options(**o)
 
@@ -55,7 +55,8 @@ def test_create_destroy_and_option_errors():
     assert h.value
     assert L.ptir_destroy(h) == 0
     for bad, status in [(dict(kernel_degree=6.0), 2), (dict(kbuffer=12), 1), (dict(visualize_lights=1), 2),
-                        (dict(min_response=0.0), 1), (dict(sph_max_degree=4), 2), (dict(wave_traversal=2), 1)]:
+                        (dict(min_response=0.0), 1), (dict(sph_max_degree=4), 2), (dict(wave_traversal=2), 1),
+                        (dict(schedule=2), 1)]:
         h = C.c_void_p()
         assert L.ptir_create(C.byref(_opts(**bad)), C.byref(h)) == status
         assert not h.value
