@@ -203,7 +203,8 @@ def time_inversion_workload(name: str, steps: int, warmup: int) -> dict:
     gradient, backward, Adam) on a workload, captured as one graph and replayed."""
     import torch
 
-    from ptir_amd import scenes
+    from ptir_amd import envmap, scenes
+    from ptir_amd.autotune import autotune
     from ptir_amd.inversion import InversionStep
     from ptir_amd.tracer import HipTracer, RaggedBatch
 
@@ -216,7 +217,16 @@ def time_inversion_workload(name: str, steps: int, warmup: int) -> dict:
         kw["ragged"] = RaggedBatch.build(sc.extra["images"], dev)
     g = sc.to(dev)
     target = scenes.target_image(sc).to(dev)
-    step = InversionStep(HipTracer(enable_metallic=True, env_sampling="cdf"), g, target, **kw)
+    # the autotuner over k-buffer {16, 32} x {one-lane kernel, wavefront} (forward, one timed call each)
+    tab = envmap.build_cdf_table(g.env)
+    scene_args = (g.particle_density(), g.particle_material(), g.features, g.shading_normal, g.env, tab)
+    tune = autotune((g.positions, g.rotation, g.scale, g.density), scene_args, 0, g.ray_to_world, g.rays_ori,
+                    g.rays_dir, tracer_kwargs=dict(enable_metallic=True, env_sampling="cdf"),
+                    trace_kwargs=dict(max_bounces=sc.max_bounces, **kw), backward=False, reps=1, kbuffers=(16, 32),
+                    traversals=("wave",))
+    del tab, scene_args
+    step = InversionStep(HipTracer(enable_metallic=True, env_sampling="cdf", **tune.tracer_kwargs()), g, target,
+                         **kw)
     step.capture()
     for _ in range(warmup):
         step.replay()
@@ -229,7 +239,8 @@ def time_inversion_workload(name: str, steps: int, warmup: int) -> dict:
     paths = sc.num_paths
     out = {"gaussians": sc.num_gaussians, "paths_per_step": paths, "max_bounces": sc.max_bounces,
            "rr_start": kw["rr_start"], "ms_per_step": round(dt * 1e3, 3), "path_samples_per_s": round(paths / dt, 1),
-           "steps": steps, "graph": True, "scene_generation_s": round(gen_s, 1), "loss_last": round(step.loss(), 6)}
+           "steps": steps, "graph": True, "scene_generation_s": round(gen_s, 1), "loss_last": round(step.loss(), 6),
+           "tracer": tune.tracer_kwargs(), "autotune": tune.as_dict()}
     if name == "batched":
         out["images"] = len(sc.extra["images"])
     del step, g
@@ -242,7 +253,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--kbuffer", type=int, default=16)
+    ap.add_argument("--kbuffer", type=int, default=0, help="k-buffer depth (0: the autotuner's choice)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="use k-buffer 16, per-wave primary traversal and the one-lane kernel instead of tuning")
     ap.add_argument("--cpu-stride", type=int, default=8, help="oracle path sample stride for cpu_baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tape", action="store_true", help="backward re-traces every segment (reference behaviour)")
@@ -272,6 +285,7 @@ def main() -> int:
     import torch
 
     from ptir_amd import envmap, scenes
+    from ptir_amd.autotune import autotune
     from ptir_amd.tracer import HipTracer
 
     dev = torch.device("cuda:0")
@@ -280,12 +294,22 @@ def main() -> int:
     g = scene.to(dev)
     tab = table.to(dev)
     target = scenes.target_image(scene).to(dev)
-    tracer = HipTracer(enable_metallic=True, kbuffer=args.kbuffer)
     paths = scene.num_paths
     dens, mat = g.particle_density(), g.particle_material()
     fargs = (scene.frame, g.ray_to_world, g.rays_ori, g.rays_dir, dens, mat, g.features, g.shading_normal, g.env, tab)
     kw = dict(render_opts=0, sph_degree=scene.sph_degree, min_transmittance=scene.min_transmittance,
               max_bounces=scene.max_bounces)
+    tuned = None
+    tcfg = dict(kbuffer=16, traversal="wave", schedule="megakernel")
+    if not args.no_autotune:
+        print("autotuning the object-scale step", file=sys.stderr, flush=True)
+        tuned = autotune((g.positions, g.rotation, g.scale, g.density), fargs[4:], scene.frame, g.ray_to_world,
+                         g.rays_ori, g.rays_dir, tracer_kwargs=dict(enable_metallic=True), trace_kwargs=kw,
+                         backward=True, reps=2)
+        tcfg = tuned.tracer_kwargs()
+    if args.kbuffer:
+        tcfg["kbuffer"] = args.kbuffer
+    tracer = HipTracer(enable_metallic=True, **tcfg)
     ev = {"fwd": [], "bwd": []}
 
     tape = None if args.no_tape else tracer.new_tape(g.rays_dir, scene.max_bounces)
@@ -325,7 +349,8 @@ def main() -> int:
     fb = fwd_bytes(cf) + (0 if tape is None else cf["paths"] * B_TAPE_PATH + cf["segments"] * B_TAPE_SEG +
                           cf["accepted"] * B_TAPE_HIT)
     bb = bwd_bytes(cb) if tape is None else bwd_tape_bytes(cb)
-    dom, ms_dom, by = ("k_trace_bwd", ms_bwd, bb) if ms_bwd >= ms_fwd else ("k_trace_fwd", ms_fwd, fb)
+    fwd_name = "k_trace_fwd" if tcfg["schedule"] == "megakernel" else "k_wf_primary+k_wf_nee+k_wf_bounce"
+    dom, ms_dom, by = ("k_trace_bwd", ms_bwd, bb) if ms_bwd >= ms_fwd else (fwd_name, ms_fwd, fb)
     achieved = by / (ms_dom * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": read_traffic(dom),
@@ -338,8 +363,9 @@ def main() -> int:
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "object-scale: 300k Gaussians, 4 views 800x800, 1 spp, maxBounces 3, env 512x256, "
                                "plain tracer fwd+bwd (ptir-mode backward); radiance-cache comparison in nrc",
-                   "gaussians": scene.num_gaussians, "paths_per_step": paths, "kbuffer": args.kbuffer,
-                   "backward": "re-trace" if tape is None else "path tape"},
+                   "gaussians": scene.num_gaussians, "paths_per_step": paths, **tcfg,
+                   "backward": "re-trace" if tape is None else "path tape",
+                   "autotune": None if tuned is None else tuned.as_dict()},
         "roofline": roofline,
         "work_units": {"fwd": cf, "bwd": cb},
     }

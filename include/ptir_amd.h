@@ -213,6 +213,14 @@ ptir_status ptir_destroy(ptir_tracer* tracer);
 ptir_status ptir_build_bvh(ptir_tracer* tracer, const float* pos, const float* rot, const float* scl, const float* dns,
                            uint32_t num_gaussians, uint32_t rebuild, int32_t allow_update, void* stream);
 
+/* Reports a traversal-stack overflow in any launch since the last report.  No counterpart in
+ * the reference, whose traversal is OptiX's own (its pipelines are compiled with exceptions
+ * off, optixTracer.cpp:399).  The kernels set a sticky device flag; this call copies it back
+ * (synchronising the stream), clears it and returns PTIR_ERR_BVH_DEPTH when it was set.
+ * A counting launch (counters != NULL) reports and clears it too; a rebuild clears it.
+ * The launch that overflowed dropped the subtrees it could not push. */
+ptir_status ptir_check(ptir_tracer* tracer, void* stream);
+
 /* OptixTracer::trace (optixTracer.cpp:909-1013).  counters may be NULL; when not NULL
  * the kernels count work units into device memory and the call copies them back
  * synchronously (use only for measurement). */

@@ -22,7 +22,7 @@ PTIR_STATUS = {
     6: "PTIR_ERR_OUT_OF_MEMORY",
 }
 
-EXPORTED = ["ptir_create", "ptir_destroy", "ptir_build_bvh", "ptir_trace", "ptir_trace_bwd", "ptir_tape_bytes",
+EXPORTED = ["ptir_create", "ptir_destroy", "ptir_build_bvh", "ptir_check", "ptir_trace", "ptir_trace_bwd", "ptir_tape_bytes",
             "ptir_last_error", "ptir_version", "ptir_cache_create", "ptir_cache_destroy", "ptir_cache_num_weights",
             "ptir_cache_table_floats", "ptir_cache_read_params", "ptir_cache_write_params", "ptir_cache_read_grads",
             "ptir_cache_query", "ptir_cache_backprop", "ptir_cache_step", "ptir_cache_train", "ptir_cache_last_error",
@@ -125,6 +125,7 @@ def lib() -> C.CDLL:
                                      C.POINTER(BwdOutputs), C.POINTER(Counters), C.c_void_p]
         L.ptir_trace_bwd_exact.argtypes = [C.c_void_p, C.POINTER(Scene), C.POINTER(Launch), C.POINTER(BwdInputs),
                                            C.POINTER(ExactGrads), C.POINTER(BwdOutputs), C.c_void_p]
+        L.ptir_check.argtypes = [C.c_void_p, C.c_void_p]
         L.ptir_tape_bytes.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_uint32, C.c_uint32]
         L.ptir_tape_bytes.restype = C.c_uint64
         L.ptir_last_error.argtypes = [C.c_void_p]
@@ -154,7 +155,7 @@ def lib() -> C.CDLL:
         L.ptir_resolve_backward.argtypes = [C.POINTER(Resolve)] + [vp] * 10
         L.ptir_adam_step.argtypes = [C.POINTER(AdamGroup), i32, C.c_float, C.c_float, C.c_float, i32, vp, vp]
         L.ptir_ragged_layout.argtypes = [vp, i32, vp, vp, vp]
-        for name in ("ptir_create", "ptir_destroy", "ptir_build_bvh", "ptir_trace", "ptir_trace_bwd", "ptir_trace_bwd_exact",
+        for name in ("ptir_create", "ptir_destroy", "ptir_build_bvh", "ptir_check", "ptir_trace", "ptir_trace_bwd", "ptir_trace_bwd_exact",
                      "ptir_cache_create", "ptir_cache_destroy", "ptir_cache_read_params", "ptir_cache_write_params",
                      "ptir_cache_read_grads", "ptir_cache_query", "ptir_cache_backprop", "ptir_cache_step", "ptir_cache_train",
                      "ptir_env_table_build", "ptir_env_table_build_noalloc", "ptir_camera_rays", "ptir_resolve_forward",

@@ -1,0 +1,101 @@
+"""Launch-configuration autotuner (no counterpart in the reference, whose OptiX programs have one
+configuration).
+
+For one workload -- the scene tensors and launch arguments of a trace -- it times every
+candidate configuration of the tracer and returns the fastest:
+
+* ``kbuffer``    k-buffer depth {8, 16, 32}: hits collected per traversal chunk.  Fewer means a
+                 tighter cull but more restarts of the walk; more means fewer restarts but a
+                 larger sorted buffer in registers;
+* ``traversal``  {"wave", "lane"}: primary rays of an 8x8 tile walk the hierarchy together or
+                 each on its own;
+* ``schedule``   {"megakernel", "wavefront"}: one lane per path, or stages over compacted
+                 queues of live paths.
+
+All candidates compute the same paths (the same hits in the same order, the same decisions), so
+the choice changes speed only.  The block size is not searched: a block is two 8x8 wave tiles
+(128 lanes), and residency is set per wave by the LDS traversal stack and the registers, so the
+block size only changes the granularity of the tile-to-CU mapping.  The hierarchy has one
+Gaussian per leaf (a binary LBVH over Morton codes), which is not a search axis either.
+
+The objective is the forward (with the path tape the autograd function records) plus, when
+``backward=True``, the tape backward, timed with HIP events on the current stream after one
+untimed call per configuration.
+"""
+
+from __future__ import annotations
+
+import itertools
+from dataclasses import dataclass, field
+
+import torch
+
+from .tracer import HipTracer
+
+KBUFFERS = (8, 16, 32)
+TRAVERSALS = ("wave", "lane")
+SCHEDULES = ("megakernel", "wavefront")
+
+
+@dataclass
+class TuneResult:
+    kbuffer: int
+    traversal: str
+    schedule: str
+    ms: float
+    timings: dict = field(default_factory=dict)  # (kbuffer, traversal, schedule) -> ms
+
+    def tracer_kwargs(self) -> dict:
+        return dict(kbuffer=self.kbuffer, traversal=self.traversal, schedule=self.schedule)
+
+    def as_dict(self) -> dict:
+        return {"kbuffer": self.kbuffer, "traversal": self.traversal, "schedule": self.schedule,
+                "ms": round(self.ms, 3),
+                "timings_ms": {f"k{k}/{t}/{s}": round(v, 3) for (k, t, s), v in sorted(self.timings.items())}}
+
+
+def _time(fn, reps: int) -> float:
+    fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def autotune(geometry, scene_args, frame, ray_to_world, ray_ori, ray_dir, *, tracer_kwargs=None, trace_kwargs=None,
+             backward: bool = True, reps: int = 3, kbuffers=KBUFFERS, traversals=TRAVERSALS,
+             schedules=SCHEDULES) -> TuneResult:
+    """Time every (kbuffer, traversal, schedule) on one workload and return the fastest.
+
+    geometry: (positions, rotations, scales, densities) for build_bvh; scene_args: the six scene
+    tensors of HipTracer.trace (particle density, material, radiance, shading normal,
+    environment, environment table); tracer_kwargs: the other HipTracer options (metallic, MIS,
+    env sampling, ...); trace_kwargs: the launch options of HipTracer.trace (max_bounces,
+    render_opts, ragged, rr_start, ...)."""
+    tracer_kwargs = dict(tracer_kwargs or {})
+    trace_kwargs = dict(trace_kwargs or {})
+    max_bounces = int(trace_kwargs.get("max_bounces", 3))
+    timings = {}
+    for k, trav, sched in itertools.product(kbuffers, traversals, schedules):
+        tr = HipTracer(kbuffer=k, traversal=trav, schedule=sched, **tracer_kwargs)
+        tr.build_bvh(*geometry)
+        tape = tr.new_tape(ray_dir, max_bounces)
+
+        def step():
+            outs = tr.trace(frame, ray_to_world, ray_ori, ray_dir, *scene_args, tape=tape, **trace_kwargs)
+            if backward:
+                grad = torch.ones_like(outs[10])
+                bwd_kw = {k2: v for k2, v in trace_kwargs.items() if k2 != "counters"}
+                tr.trace_bwd(frame, ray_to_world, ray_ori, ray_dir, *outs[:8], outs[10], outs[11], *scene_args,
+                             None, None, None, None, None, None, None, None, None, None, None, None, None, None,
+                             grad, None, tape=tape, **bwd_kw)
+
+        timings[(k, trav, sched)] = _time(step, reps)
+        del tr, tape
+    best = min(timings, key=timings.get)
+    return TuneResult(best[0], best[1], best[2], timings[best], timings)

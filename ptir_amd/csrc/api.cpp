@@ -142,11 +142,11 @@ ptir::DeviceScene make_scene(ptir_tracer* t, const ptir_scene* sc, const ptir_la
     return S;
 }
 
+// The traversal-stack error flag is sticky: a kernel sets it, and it is cleared only by a
+// rebuild of the hierarchy and by the call that reports it (a counting launch or ptir_check).
 ptir_status begin_counters(ptir_tracer* t, ptir::DeviceScene& S, ptir_counters* c, hipStream_t s) {
-    hipError_t e = ptir::fill_zero(t->bvh.error_flag, sizeof(int32_t), s);
-    if (e != hipSuccess) return hip_fail(t, e, "zeroing the error flag");
     if (!c) return PTIR_OK;
-    e = ptir::fill_zero(t->bvh.counters, size_t(ptir::kCounterCopies) * ptir::kCounterStride * sizeof(unsigned long long), s);
+    hipError_t e = ptir::fill_zero(t->bvh.counters, size_t(ptir::kCounterCopies) * ptir::kCounterStride * sizeof(unsigned long long), s);
     if (e != hipSuccess) return hip_fail(t, e, "zeroing counters");
     S.counters = t->bvh.counters;
     return PTIR_OK;
@@ -176,7 +176,11 @@ ptir_status end_counters(ptir_tracer* t, ptir_counters* c, hipStream_t s) {
     c->traversals = h[ptir::C_TRAVERSALS];
     c->nodes = h[ptir::C_NODES];
     c->leaf_tests = h[ptir::C_LEAF_TESTS];
-    if (flag) return fail(t, PTIR_ERR_BVH_DEPTH, "traversal stack overflow");
+    if (flag) {
+        e = ptir::fill_zero(t->bvh.error_flag, sizeof(int32_t), s);
+        if (e != hipSuccess) return hip_fail(t, e, "clearing the error flag");
+        return fail(t, PTIR_ERR_BVH_DEPTH, "traversal stack overflow");
+    }
     return PTIR_OK;
 }
 
@@ -240,7 +244,8 @@ ptir_status ptir_build_bvh(ptir_tracer* t, const float* pos, const float* rot, c
     if (e != hipSuccess) return hip_fail(t, e, "bvh_build");
     if (rebuild) {
         int32_t depth = 0;
-        e = ptir::bvh_depth(t->bvh, s, &depth);
+        e = ptir::fill_zero(t->bvh.error_flag, sizeof(int32_t), s);
+        if (e == hipSuccess) e = ptir::bvh_depth(t->bvh, s, &depth);
         if (e != hipSuccess) return hip_fail(t, e, "bvh_depth");
         if (depth + 1 > ptir::kStackDeep)
             return fail(t, PTIR_ERR_BVH_DEPTH, "hierarchy depth " + std::to_string(depth) + " exceeds the traversal stack");
@@ -248,6 +253,20 @@ ptir_status ptir_build_bvh(ptir_tracer* t, const float* pos, const float* rot, c
     }
     t->built = true;
     return PTIR_OK;
+}
+
+ptir_status ptir_check(ptir_tracer* t, void* stream) {
+    if (!t) return fail(nullptr, PTIR_ERR_INVALID_ARGUMENT, "null tracer");
+    if (!t->bvh.error_flag) return PTIR_OK;  // nothing built, nothing traced
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int32_t flag = 0;
+    hipError_t e = hipMemcpyAsync(&flag, t->bvh.error_flag, sizeof(flag), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(t, e, "reading the error flag");
+    if (!flag) return PTIR_OK;
+    e = ptir::fill_zero(t->bvh.error_flag, sizeof(int32_t), s);
+    if (e != hipSuccess) return hip_fail(t, e, "clearing the error flag");
+    return fail(t, PTIR_ERR_BVH_DEPTH, "a traversal stack overflowed in a launch since the last check");
 }
 
 ptir_status ptir_trace(ptir_tracer* t, const ptir_scene* sc, const ptir_launch* la, const ptir_fwd_outputs* out,

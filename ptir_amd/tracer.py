@@ -154,6 +154,11 @@ class HipTracer:
     def __del__(self):
         self.close()
 
+    def check(self) -> None:
+        """Raise PtirError (PTIR_ERR_BVH_DEPTH) when a traversal stack overflowed in any launch
+        since the last check; synchronises the current stream.  ptir_check."""
+        _lib.check(_lib.lib().ptir_check(self._h, _stream()), self._h)
+
     # -------------------------------------------------------------- build_bvh
     def build_bvh(self, mog_pos, mog_rot, mog_scl, mog_dns, rebuild=True, allow_update=False):
         """OptixTracer::buildBVH (optixTracer.cpp:633-907)."""
@@ -582,7 +587,12 @@ class Tracer:
             _conf_get(r, "particle_radiance_sph_degree", 3), _conf_get(r, "enable_normals", True),
             _conf_get(r, "enable_hitcounts", True), _conf_get(r, "enable_mis", True), bool(metallic),
             self.visualize_lights, kbuffer=int(_conf_get(r, "kbuffer_depth", 16)),
-            env_sampling=str(_conf_get(r, "env_sampling", "alias")))
+            env_sampling=str(_conf_get(r, "env_sampling", "alias")),
+            traversal=str(_conf_get(r, "primary_traversal", "wave")),
+            schedule=str(_conf_get(r, "schedule", "megakernel")))
+        # report a traversal-stack overflow after each render (one stream synchronisation;
+        # skipped while a graph is being captured)
+        self.check_errors = bool(_conf_get(r, "check_errors", True))
         self.updates_since_rebuild = 0
         self._built_geometry = None
         self._spp_fallback_warned = False
@@ -717,6 +727,8 @@ class Tracer:
                if res.get(k) is not None and (keep is None or k in keep)}
         out["frame_time_ms"] = 0.0
         out["mog_visibility"] = visibility
+        if self.check_errors and not torch.cuda.is_current_stream_capturing():
+            self.tracer_wrapper.check()
         return render.apply_gt_mask(out, gpu_batch)
 
     def _resolve(self, mean: dict) -> dict:

@@ -278,6 +278,7 @@ def test_object_scene_full_size_properties(device):
     # hit Gaussians get gradients, never-visible ones none
     nz = (gr[1].abs().sum(1) > 0)
     assert bool((~(vis != 0) & nz).sum() == 0)
+    tr.check()  # no traversal stack overflowed on the 300k-Gaussian hierarchy
 
 
 def test_graph_capture_replays_match_eager(small, device):
