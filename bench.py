@@ -200,7 +200,8 @@ def nrc_comparison(train_steps: int = 150, timed_steps: int = 5, spps=(1, 4, 16,
 
 def time_inversion_workload(name: str, steps: int, warmup: int) -> dict:
     """Steps/s of the whole inversion step (device env table, BVH refit, forward, loss
-    gradient, backward, Adam) on a workload, captured as one graph and replayed."""
+    gradient, backward, Adam) on a workload, captured as one graph and replayed (the capture
+    first fits the tape's per-path hit capacity so that the backward re-traces nothing)."""
     import torch
 
     from ptir_amd import envmap, scenes
@@ -240,7 +241,8 @@ def time_inversion_workload(name: str, steps: int, warmup: int) -> dict:
     out = {"gaussians": sc.num_gaussians, "paths_per_step": paths, "max_bounces": sc.max_bounces,
            "rr_start": kw["rr_start"], "ms_per_step": round(dt * 1e3, 3), "path_samples_per_s": round(paths / dt, 1),
            "steps": steps, "graph": True, "scene_generation_s": round(gen_s, 1), "loss_last": round(step.loss(), 6),
-           "tracer": tune.tracer_kwargs(), "autotune": tune.as_dict()}
+           "tracer": tune.tracer_kwargs(), "autotune": tune.as_dict(),
+           "tape_hits_per_path": step.tracer.tape_hits_per_path, "tape_gb": round(step.tape.numel() / 1e9, 2)}
     if name == "batched":
         out["images"] = len(sc.extra["images"])
     del step, g
