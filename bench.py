@@ -218,12 +218,13 @@ def time_inversion_workload(name: str, steps: int, warmup: int) -> dict:
         kw["ragged"] = RaggedBatch.build(sc.extra["images"], dev)
     g = sc.to(dev)
     target = scenes.target_image(sc).to(dev)
-    # the autotuner over k-buffer {16, 32} x {one-lane kernel, wavefront} (forward, one timed call each)
+    # the autotuner over {one-lane kernel, wavefront} x {LBVH, PLOC} at k-buffer 16 (build + forward,
+    # one timed call each; k-buffer 32 and per-lane primary traversal lose on every workload here)
     tab = envmap.build_cdf_table(g.env)
     scene_args = (g.particle_density(), g.particle_material(), g.features, g.shading_normal, g.env, tab)
     tune = autotune((g.positions, g.rotation, g.scale, g.density), scene_args, 0, g.ray_to_world, g.rays_ori,
                     g.rays_dir, tracer_kwargs=dict(enable_metallic=True, env_sampling="cdf"),
-                    trace_kwargs=dict(max_bounces=sc.max_bounces, **kw), backward=False, reps=1, kbuffers=(16, 32),
+                    trace_kwargs=dict(max_bounces=sc.max_bounces, **kw), backward=False, reps=1, kbuffers=(16,),
                     traversals=("wave",))
     del tab, scene_args
     step = InversionStep(HipTracer(enable_metallic=True, env_sampling="cdf", **tune.tracer_kwargs()), g, target,
@@ -302,7 +303,7 @@ def main() -> int:
     kw = dict(render_opts=0, sph_degree=scene.sph_degree, min_transmittance=scene.min_transmittance,
               max_bounces=scene.max_bounces)
     tuned = None
-    tcfg = dict(kbuffer=16, traversal="wave", schedule="megakernel")
+    tcfg = dict(kbuffer=16, traversal="wave", schedule="megakernel", builder="lbvh")
     if not args.no_autotune:
         print("autotuning the object-scale step", file=sys.stderr, flush=True)
         tuned = autotune((g.positions, g.rotation, g.scale, g.density), fargs[4:], scene.frame, g.ray_to_world,

@@ -57,6 +57,12 @@ typedef struct ptir_options {
                                      per-path decisions and outputs.  The tracer then keeps a device
                                      workspace of 164 bytes per path, grown on demand (by a host-
                                      synchronous allocation: make the first call outside graph capture) */
+    int32_t builder;              /* hierarchy builder (no counterpart in the reference, whose hierarchy
+                                     is OptiX's): 0 = LBVH (radix splits of the Morton order); 1 = PLOC
+                                     (bottom-up clustering of the same Morton-ordered leaves by smallest
+                                     union surface area; fewer nodes visited per ray, a longer build with
+                                     a host synchronisation every 8 clustering rounds).  The trace results
+                                     do not depend on it; a refit keeps the tree either builder made */
 } ptir_options;
 
 /* Scene inputs of one trace call (OptixTracer::trace argument list,

@@ -35,7 +35,8 @@ class Options(C.Structure):
     _fields_ = [("kernel_degree", C.c_float), ("min_response", C.c_float), ("density_clamping", C.c_int32),
                 ("sph_max_degree", C.c_int32), ("enable_normals", C.c_int32), ("enable_hitcounts", C.c_int32),
                 ("enable_mis", C.c_int32), ("enable_metallic", C.c_int32), ("visualize_lights", C.c_int32),
-                ("kbuffer", C.c_int32), ("wave_traversal", C.c_int32), ("schedule", C.c_int32)]
+                ("kbuffer", C.c_int32), ("wave_traversal", C.c_int32), ("schedule", C.c_int32),
+                ("builder", C.c_int32)]
 
 
 class Scene(C.Structure):

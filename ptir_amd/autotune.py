@@ -10,17 +10,20 @@ candidate configuration of the tracer and returns the fastest:
 * ``traversal``  {"wave", "lane"}: primary rays of an 8x8 tile walk the hierarchy together or
                  each on its own;
 * ``schedule``   {"megakernel", "wavefront"}: one lane per path, or stages over compacted
-                 queues of live paths.
+                 queues of live paths;
+* ``builder``    {"lbvh", "ploc"}: the hierarchy over the same Morton-ordered leaves, by radix
+                 splits of the codes or by bottom-up clustering of smallest union area (fewer
+                 nodes per ray, a longer build; the timed step includes the build).
 
 All candidates compute the same paths (the same hits in the same order, the same decisions), so
 the choice changes speed only.  The block size is not searched: a block is two 8x8 wave tiles
 (128 lanes), and residency is set per wave by the LDS traversal stack and the registers, so the
-block size only changes the granularity of the tile-to-CU mapping.  The hierarchy has one
-Gaussian per leaf (a binary LBVH over Morton codes), which is not a search axis either.
+block size only changes the granularity of the tile-to-CU mapping.  Leaves hold one Gaussian
+each (the candidate test is the leaf test), which is not a search axis either.
 
-The objective is the forward (with the path tape the autograd function records) plus, when
-``backward=True``, the tape backward, timed with HIP events on the current stream after one
-untimed call per configuration.
+The objective is the hierarchy build plus the forward (with the path tape the autograd
+function records) plus, when ``backward=True``, the tape backward, timed with HIP events on the
+current stream after one untimed call per configuration.
 """
 
 from __future__ import annotations
@@ -35,6 +38,7 @@ from .tracer import HipTracer
 KBUFFERS = (8, 16, 32)
 TRAVERSALS = ("wave", "lane")
 SCHEDULES = ("megakernel", "wavefront")
+BUILDERS = ("lbvh", "ploc")
 
 
 @dataclass
@@ -42,16 +46,17 @@ class TuneResult:
     kbuffer: int
     traversal: str
     schedule: str
+    builder: str
     ms: float
-    timings: dict = field(default_factory=dict)  # (kbuffer, traversal, schedule) -> ms
+    timings: dict = field(default_factory=dict)  # (kbuffer, traversal, schedule, builder) -> ms
 
     def tracer_kwargs(self) -> dict:
-        return dict(kbuffer=self.kbuffer, traversal=self.traversal, schedule=self.schedule)
+        return dict(kbuffer=self.kbuffer, traversal=self.traversal, schedule=self.schedule, builder=self.builder)
 
     def as_dict(self) -> dict:
         return {"kbuffer": self.kbuffer, "traversal": self.traversal, "schedule": self.schedule,
-                "ms": round(self.ms, 3),
-                "timings_ms": {f"k{k}/{t}/{s}": round(v, 3) for (k, t, s), v in sorted(self.timings.items())}}
+                "builder": self.builder, "ms": round(self.ms, 3),
+                "timings_ms": {f"k{k}/{t}/{s}/{b}": round(v, 3) for (k, t, s, b), v in sorted(self.timings.items())}}
 
 
 def _time(fn, reps: int) -> float:
@@ -69,8 +74,8 @@ def _time(fn, reps: int) -> float:
 
 def autotune(geometry, scene_args, frame, ray_to_world, ray_ori, ray_dir, *, tracer_kwargs=None, trace_kwargs=None,
              backward: bool = True, reps: int = 3, kbuffers=KBUFFERS, traversals=TRAVERSALS,
-             schedules=SCHEDULES) -> TuneResult:
-    """Time every (kbuffer, traversal, schedule) on one workload and return the fastest.
+             schedules=SCHEDULES, builders=BUILDERS) -> TuneResult:
+    """Time every (kbuffer, traversal, schedule, builder) on one workload and return the fastest.
 
     geometry: (positions, rotations, scales, densities) for build_bvh; scene_args: the six scene
     tensors of HipTracer.trace (particle density, material, radiance, shading normal,
@@ -81,12 +86,12 @@ def autotune(geometry, scene_args, frame, ray_to_world, ray_ori, ray_dir, *, tra
     trace_kwargs = dict(trace_kwargs or {})
     max_bounces = int(trace_kwargs.get("max_bounces", 3))
     timings = {}
-    for k, trav, sched in itertools.product(kbuffers, traversals, schedules):
-        tr = HipTracer(kbuffer=k, traversal=trav, schedule=sched, **tracer_kwargs)
-        tr.build_bvh(*geometry)
+    for k, trav, sched, bld in itertools.product(kbuffers, traversals, schedules, builders):
+        tr = HipTracer(kbuffer=k, traversal=trav, schedule=sched, builder=bld, **tracer_kwargs)
         tape = tr.new_tape(ray_dir, max_bounces)
 
         def step():
+            tr.build_bvh(*geometry)
             outs = tr.trace(frame, ray_to_world, ray_ori, ray_dir, *scene_args, tape=tape, **trace_kwargs)
             if backward:
                 grad = torch.ones_like(outs[10])
@@ -95,7 +100,7 @@ def autotune(geometry, scene_args, frame, ray_to_world, ray_ori, ray_dir, *, tra
                              None, None, None, None, None, None, None, None, None, None, None, None, None, None,
                              grad, None, tape=tape, **bwd_kw)
 
-        timings[(k, trav, sched)] = _time(step, reps)
+        timings[(k, trav, sched, bld)] = _time(step, reps)
         del tr, tape
     best = min(timings, key=timings.get)
-    return TuneResult(best[0], best[1], best[2], timings[best], timings)
+    return TuneResult(best[0], best[1], best[2], best[3], timings[best], timings)

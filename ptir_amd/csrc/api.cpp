@@ -210,6 +210,8 @@ ptir_status ptir_create(const ptir_options* options, ptir_tracer** out) {
         return fail(nullptr, PTIR_ERR_INVALID_ARGUMENT, "wave_traversal must be 0 or 1");
     if (options->schedule != 0 && options->schedule != 1)
         return fail(nullptr, PTIR_ERR_INVALID_ARGUMENT, "schedule must be 0 (one lane per path) or 1 (wavefront)");
+    if (options->builder != 0 && options->builder != 1)
+        return fail(nullptr, PTIR_ERR_INVALID_ARGUMENT, "builder must be 0 (LBVH) or 1 (PLOC)");
     ptir_tracer* t = new (std::nothrow) ptir_tracer();
     if (!t) return fail(nullptr, PTIR_ERR_OUT_OF_MEMORY, "host allocation failed");
     t->opts = *options;
@@ -240,7 +242,7 @@ ptir_status ptir_build_bvh(ptir_tracer* t, const float* pos, const float* rot, c
         t->built = true;
         return PTIR_OK;
     }
-    hipError_t e = ptir::bvh_build(t->bvh, kopts(t->opts), pos, rot, scl, dns, n, rebuild != 0, s);
+    hipError_t e = ptir::bvh_build(t->bvh, kopts(t->opts), pos, rot, scl, dns, n, rebuild != 0, t->opts.builder, s);
     if (e != hipSuccess) return hip_fail(t, e, "bvh_build");
     if (rebuild) {
         int32_t depth = 0;

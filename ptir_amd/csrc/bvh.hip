@@ -197,12 +197,67 @@ __global__ void __launch_bounds__(256) k_refit(const float4* __restrict__ aabb, 
     }
 }
 
+// Depth of the deepest leaf; the walk stops at kDepthCap, which the build then rejects
+// (a malformed parent chain cannot loop).
+constexpr int kDepthCap = 4096;
 __global__ void __launch_bounds__(256) k_depth(const int32_t* __restrict__ parent, int n, int32_t* depth) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     int d = 0;
-    for (int p = parent[(n - 1) + j]; p >= 0; p = parent[p]) ++d;
+    for (int p = parent[(n - 1) + j]; p >= 0 && d < kDepthCap; p = parent[p]) ++d;
     atomicMax(depth, d);
+}
+
+// ---------------------------------------------------------------- PLOC builder (ploc.hpp)
+__global__ void __launch_bounds__(256) k_ploc_init(const float4* __restrict__ aabb, const uint32_t* __restrict__ leaf_id,
+                                                   int n, int32_t* __restrict__ c, float4* __restrict__ b,
+                                                   ploc::State* __restrict__ st, int32_t* __restrict__ parent) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t g = leaf_id[j];
+    c[j] = ~j;
+    b[2 * j] = aabb[2 * g];
+    b[2 * j + 1] = aabb[2 * g + 1];
+    if (j == 0) {
+        st[0] = ploc::State{n, n - 2};
+        parent[0] = -1;  // node 0 is the last merge, the root
+    }
+}
+__global__ void __launch_bounds__(256) k_ploc_nearest(ploc::Iter it) {
+    ploc::nearest(it, int(blockIdx.x * blockDim.x + threadIdx.x));
+}
+__global__ void __launch_bounds__(256) k_ploc_flags(ploc::Iter it) {
+    ploc::flags(it, int(blockIdx.x * blockDim.x + threadIdx.x));
+}
+__global__ void __launch_bounds__(256) k_ploc_compact(ploc::Iter it) {
+    ploc::compact(it, int(blockIdx.x * blockDim.x + threadIdx.x));
+}
+// Structure check of a finished tree, one thread per element (n-1 nodes, then n leaves):
+// every internal child of node k has an index above k (made earlier), so the child links have
+// no cycle, and parent and child links agree.  Any violation sets *bad.
+__global__ void __launch_bounds__(256) k_ploc_check(const float4* __restrict__ nodes,
+                                                    const int32_t* __restrict__ parent, int n,
+                                                    int32_t* __restrict__ bad) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= 2 * n - 1) return;
+    bool ok = true;
+    if (x < n - 1) {
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const int ch = __float_as_int(nodes[4 * size_t(x) + s].w);
+            if (ch >= 0) ok = ok && ch > x && ch < n - 1 && parent[ch] == x;
+            else ok = ok && ~ch < n && parent[(n - 1) + ~ch] == x;
+        }
+    }
+    if (x != 0) {
+        const int p = parent[x];
+        const int self = x < n - 1 ? x : ~(x - (n - 1));
+        ok = ok && p >= 0 && p < n - 1 &&
+             (__float_as_int(nodes[4 * size_t(p)].w) == self || __float_as_int(nodes[4 * size_t(p) + 1].w) == self);
+    } else {
+        ok = ok && parent[0] == -1;
+    }
+    if (!ok) atomicOr(bad, 1);
 }
 
 __global__ void k_clear_flags(uint32_t* flags, int m) {
@@ -316,7 +371,8 @@ hipError_t bvh_reserve(BvhBuffers& b, uint32_t n) {
 void bvh_free(BvhBuffers& b) {
     void* ptrs[] = {b.aabb, b.keys, b.keys_sorted, b.idx, b.leaf_id, b.nodes, b.parent, b.flags, b.bounds_i,
                     b.bounds, b.depth, b.sort_tmp, b.grec, b.srec, b.prec, b.dmat_leaf, b.counters, b.error_flag,
-                    b.tape_state};
+                    b.tape_state, b.pc[0], b.pc[1], b.pb[0], b.pb[1], b.pnn, b.pval, b.pscan, b.pstate, b.pbad,
+                    b.scan_tmp};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     b = BvhBuffers();
@@ -324,8 +380,77 @@ void bvh_free(BvhBuffers& b) {
 
 static inline unsigned blocks(uint32_t n, unsigned bs) { return (n + bs - 1) / bs; }
 
+// PLOC over the sorted leaves: nodes, boxes and parent links of the whole tree.  The host
+// reads the cluster count back every kPlocBatch iterations (a rebuild synchronises anyway, for
+// the depth check).  Returns hipSuccess with *converged = false when kPlocMaxIter iterations
+// did not finish or the finished tree fails k_ploc_check (the caller then builds the LBVH).
+constexpr int kPlocBatch = 8;
+constexpr int kPlocMaxIter = 4096;
+
+static hipError_t ploc_reserve(BvhBuffers& b, uint32_t n) {
+    if (n <= b.pcap && b.pc[0]) return hipSuccess;
+    hipError_t e;
+    const size_t N = n;
+#define G(ptr, cnt) if ((e = grow(ptr, cnt)) != hipSuccess) return e;
+    G(b.pc[0], N);
+    G(b.pc[1], N);
+    G(b.pb[0], 2 * N);
+    G(b.pb[1], 2 * N);
+    G(b.pnn, N);
+    G(b.pval, N);
+    G(b.pscan, N);
+    G(b.pstate, 2);
+    G(b.pbad, 1);
+#undef G
+    size_t tmp = 0;
+    if ((e = hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, b.pval, b.pscan, int(n))) != hipSuccess) return e;
+    if (b.scan_tmp) (void)hipFree(b.scan_tmp);
+    b.scan_tmp = nullptr;
+    if ((e = hipMalloc(&b.scan_tmp, tmp + 256)) != hipSuccess) return e;
+    b.scan_tmp_bytes = tmp + 256;
+    b.pcap = n;
+    return hipSuccess;
+}
+
+static hipError_t ploc_build(BvhBuffers& b, uint32_t n, hipStream_t s, bool* converged) {
+    *converged = false;
+    hipError_t e = ploc_reserve(b, n);
+    if (e != hipSuccess) return e;
+    k_ploc_init<<<blocks(n, 256), 256, 0, s>>>(b.aabb, b.leaf_id, int(n), b.pc[0], b.pb[0], b.pstate, b.parent);
+    int known = int(n);  // an upper bound of the cluster count (it only decreases)
+    for (int it = 0; it < kPlocMaxIter;) {
+        for (int k = 0; k < kPlocBatch; ++k, ++it) {
+            const int a = it & 1, z = a ^ 1;
+            const ploc::Iter I{b.pc[a], b.pb[a], b.pc[z], b.pb[z], b.pnn, b.pval, b.pscan, b.pstate + a, b.pstate + z,
+                               b.nodes, b.parent, int(n)};
+            const unsigned grid = blocks(uint32_t(known), 256);
+            k_ploc_nearest<<<grid, 256, 0, s>>>(I);
+            k_ploc_flags<<<grid, 256, 0, s>>>(I);
+            size_t tmp = b.scan_tmp_bytes;
+            if ((e = hipcub::DeviceScan::ExclusiveSum(b.scan_tmp, tmp, b.pval, b.pscan, known, s)) != hipSuccess)
+                return e;
+            k_ploc_compact<<<grid, 256, 0, s>>>(I);
+        }
+        ploc::State h{0, 0};
+        if ((e = hipMemcpyAsync(&h, b.pstate + (it & 1), sizeof(h), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+        if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+        if (h.cnt <= 1) {
+            if (h.cnt != 1 || h.base != -1) return hipSuccess;
+            int32_t bad = 1;
+            if ((e = fill_zero(b.pbad, sizeof(int32_t), s)) != hipSuccess) return e;
+            k_ploc_check<<<blocks(2 * n - 1, 256), 256, 0, s>>>(b.nodes, b.parent, int(n), b.pbad);
+            if ((e = hipMemcpyAsync(&bad, b.pbad, sizeof(bad), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+            if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+            *converged = bad == 0;
+            return hipSuccess;
+        }
+        known = h.cnt;
+    }
+    return hipSuccess;
+}
+
 hipError_t bvh_build(BvhBuffers& b, const KernelOpts& o, const float* pos, const float* rot, const float* scl,
-                     const float* dns, uint32_t n, bool rebuild, hipStream_t s) {
+                     const float* dns, uint32_t n, bool rebuild, int builder, hipStream_t s) {
     hipError_t e = bvh_reserve(b, n);
     if (e != hipSuccess) return e;
     b.n = n;
@@ -336,6 +461,11 @@ hipError_t bvh_build(BvhBuffers& b, const KernelOpts& o, const float* pos, const
         e = hipcub::DeviceRadixSort::SortPairs(b.sort_tmp, b.sort_tmp_bytes, b.keys, b.keys_sorted, b.idx, b.leaf_id,
                                                int(n), 0, 63, s);
         if (e != hipSuccess) return e;
+        if (n > 1 && builder == 1) {
+            bool done = false;
+            if ((e = ploc_build(b, n, s, &done)) != hipSuccess) return e;
+            if (done) return hipGetLastError();  // nodes, boxes and parents are written
+        }
         if (n > 1) k_karras<<<blocks(n - 1, 256), 256, 0, s>>>(b.keys_sorted, int(n), b.nodes, b.parent, b.flags);
     } else if (n > 1) {
         k_clear_flags<<<blocks(n - 1, 256), 256, 0, s>>>(b.flags, int(n - 1));

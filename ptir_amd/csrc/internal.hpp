@@ -8,6 +8,7 @@
 #include <string>
 
 #include "device.hpp"
+#include "ploc.hpp"
 
 namespace ptir {
 
@@ -41,12 +42,23 @@ struct BvhBuffers {
     unsigned long long* counters = nullptr;  // [kCounterCopies * kCounterStride]
     int32_t* error_flag = nullptr;   // [1]
     int32_t* tape_state = nullptr;   // [1]
+    // PLOC builder work buffers (allocated on its first build)
+    uint32_t pcap = 0;
+    int32_t* pc[2] = {nullptr, nullptr};  // [N] cluster links, ping-pong
+    float4* pb[2] = {nullptr, nullptr};   // [2N] cluster boxes, ping-pong
+    int32_t* pnn = nullptr;               // [N]
+    unsigned long long* pval = nullptr;   // [N]
+    unsigned long long* pscan = nullptr;  // [N]
+    ploc::State* pstate = nullptr;        // [2], ping-pong
+    int32_t* pbad = nullptr;              // [1] structure check result
+    void* scan_tmp = nullptr;
+    size_t scan_tmp_bytes = 0;
 };
 
 hipError_t bvh_reserve(BvhBuffers& b, uint32_t n);
 void bvh_free(BvhBuffers& b);
 hipError_t bvh_build(BvhBuffers& b, const KernelOpts& o, const float* pos, const float* rot, const float* scl,
-                     const float* dns, uint32_t n, bool rebuild, hipStream_t s);
+                     const float* dns, uint32_t n, bool rebuild, int builder, hipStream_t s);
 hipError_t bvh_depth(BvhBuffers& b, hipStream_t s, int32_t* host_depth);
 hipError_t prep_records(BvhBuffers& b, const KernelOpts& o, const float* density, const float* material,
                         const float* snormal, hipStream_t s);
@@ -101,7 +113,9 @@ hipError_t launch_trace_adjoint(const DeviceScene& S, const AdjIO& io, int kbuf,
 
 constexpr int kTraceBlock = 128;  // 2 waves: a 16x8 pixel tile
 constexpr int kStackDeep = 64;    // traversal stack entries per lane (LDS): any accepted hierarchy
-constexpr int kStackShort = 32;   // hierarchies of depth <= 32 run with this one (more waves fit)
+// hierarchies of depth < 36 run with this one: 36 words per lane of LDS (with the wave stacks,
+// 18.9 KB per 128-lane block) still fit 8 blocks = four waves per SIMD in 160 KB of LDS
+constexpr int kStackShort = 36;
 
 // bytes of a tape with `segs` segments per path and `cap` hits per path
 inline uint64_t tape_bytes(uint64_t paths, uint32_t segs, uint32_t cap) {

@@ -92,7 +92,8 @@ class HipTracer:
                  enable_normals: bool = True, enable_hitcounts: bool = True, enable_mis: bool = True,
                  enable_metallic: bool = False, visualize_lights: bool = False, kbuffer: int = 16,
                  use_tape: bool = True, tape_hits_per_path: int = 64, traversal: str = "wave",
-                 env_sampling: str = "alias", gradients: str = "ptir", schedule: str = "megakernel"):
+                 env_sampling: str = "alias", gradients: str = "ptir", schedule: str = "megakernel",
+                 builder: str = "lbvh"):
         if pipeline != "reference" or backward_pipeline not in ("referenceBwd", "reference" + "Bwd"):
             raise ValueError(f"unsupported pipeline pair ({pipeline}, {backward_pipeline}); "
                              "only the PTIR 'reference' path tracer is built")
@@ -105,6 +106,8 @@ class HipTracer:
                              f"(envmap.build_cdf_table), not {env_sampling!r}")
         if schedule not in ("megakernel", "wavefront"):
             raise ValueError(f"schedule must be 'megakernel' (one lane per path) or 'wavefront', not {schedule!r}")
+        if builder not in ("lbvh", "ploc"):
+            raise ValueError(f"builder must be 'lbvh' or 'ploc', not {builder!r}")
         if gradients not in ("ptir", "exact"):
             raise ValueError(f"gradients must be 'ptir' (the reference's backward) or 'exact', not {gradients!r}")
         # which backward the autograd function runs: the reference's path-replay backward
@@ -116,7 +119,8 @@ class HipTracer:
                          int(bool(particle_kernel_density_clamping)), int(particle_radiance_sph_degree),
                          int(bool(enable_normals)), int(bool(enable_hitcounts)), int(bool(enable_mis)),
                          int(bool(enable_metallic)), int(bool(visualize_lights)), int(kbuffer),
-                         {"lane": 0, "wave": 1}[traversal], {"megakernel": 0, "wavefront": 1}[schedule])
+                         {"lane": 0, "wave": 1}[traversal], {"megakernel": 0, "wavefront": 1}[schedule],
+                         {"lbvh": 0, "ploc": 1}[builder])
         h = C.c_void_p()
         L = _lib.lib()
         _lib.check(L.ptir_create(C.byref(o), C.byref(h)), None)
@@ -589,7 +593,8 @@ class Tracer:
             self.visualize_lights, kbuffer=int(_conf_get(r, "kbuffer_depth", 16)),
             env_sampling=str(_conf_get(r, "env_sampling", "alias")),
             traversal=str(_conf_get(r, "primary_traversal", "wave")),
-            schedule=str(_conf_get(r, "schedule", "megakernel")))
+            schedule=str(_conf_get(r, "schedule", "megakernel")),
+            builder=str(_conf_get(r, "bvh_builder", "lbvh")))
         # report a traversal-stack overflow after each render (one stream synchronisation;
         # skipped while a graph is being captured)
         self.check_errors = bool(_conf_get(r, "check_errors", True))

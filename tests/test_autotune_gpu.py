@@ -1,5 +1,5 @@
 """The launch-configuration autotuner (ptir_amd/autotune.py) on the GPU: it times every
-(k-buffer, traversal, schedule) candidate, returns the fastest, and the tracer built from its
+(k-buffer, traversal, schedule, builder) candidate, returns the fastest, and the tracer built from its
 choice writes the same outputs as the default configuration, bit for bit (the candidates
 differ in speed only: k-buffer depth and traversal change how the hits are collected, not
 which hits or their order, and the two schedules run the same device functions)."""
@@ -25,12 +25,13 @@ def test_autotune_returns_the_fastest_and_the_same_image(device):
     print("autotune small scene", flush=True)
     res = autotune(geom, args, sc.frame, g.ray_to_world, g.rays_ori, g.rays_dir,
                    tracer_kwargs=dict(enable_metallic=True), trace_kwargs=kw, backward=True, reps=1)
-    assert len(res.timings) == 12
+    assert len(res.timings) == 24
     assert all(t > 0 for t in res.timings.values())
     assert res.ms == min(res.timings.values())
-    assert res.timings[(res.kbuffer, res.traversal, res.schedule)] == res.ms
+    assert res.timings[(res.kbuffer, res.traversal, res.schedule, res.builder)] == res.ms
     d = res.as_dict()
-    assert set(d) == {"kbuffer", "traversal", "schedule", "ms", "timings_ms"} and len(d["timings_ms"]) == 12
+    assert set(d) == {"kbuffer", "traversal", "schedule", "builder", "ms", "timings_ms"}
+    assert len(d["timings_ms"]) == 24
 
     outs = []
     for tkw in (dict(), res.tracer_kwargs()):
