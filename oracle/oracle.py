@@ -62,6 +62,16 @@ class OracleBwdOut(C.Structure):
     _fields_ = [("dmat", C.c_void_p), ("denv", C.c_void_p)]
 
 
+class OracleGrtIn(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ["rad", "dns", "hit", "hit2", "distortion", "snrm", "rad_grad", "dns_grad",
+                                           "hit_grad", "hit2_grad", "distortion_grad", "snrm_grad"]] + \
+               [("inclusive_end", C.c_int32)]
+
+
+class OracleGrtOut(C.Structure):
+    _fields_ = [("ddensity", C.c_void_p), ("dsh", C.c_void_p), ("dsnormal", C.c_void_p)]
+
+
 class OracleCounters(C.Structure):
     _fields_ = [(k, C.c_int64) for k in ["paths", "segments", "candidates", "accepted", "accepted_primary",
                                           "shadow_candidates", "env_lookups", "mat_atomics", "env_atomics"]]
@@ -102,6 +112,11 @@ def lib():
                                         C.c_void_p]
         L.oracle_candidates.restype = C.c_int
         L.oracle_scene_aabb.argtypes = [C.POINTER(OracleScene), C.c_void_p]
+        L.oracle_grt_trace.argtypes = [C.POINTER(OracleScene), C.POINTER(OracleRays), C.POINTER(OracleFwdOut), C.c_int]
+        L.oracle_grt_trace_dual.argtypes = [C.POINTER(OracleScene), C.POINTER(OracleRays), C.POINTER(OracleFwdOut),
+                                            C.POINTER(OracleFwdOut), C.c_int]
+        L.oracle_grt_trace_bwd.argtypes = [C.POINTER(OracleScene), C.POINTER(OracleRays), C.POINTER(OracleGrtIn),
+                                           C.POINTER(OracleGrtOut), C.c_int]
         L.oracle_trace_dual.argtypes = [C.POINTER(OracleScene), C.POINTER(OracleRays), C.POINTER(OracleFwdOut),
                                         C.POINTER(OracleFwdOut), C.c_int]
         _lib = L
@@ -357,3 +372,77 @@ def env_cdf_table(env, h=64, w=128):
     out[1, :, 0] = marg
     out[2] = np.where(sa > 0, wts / (total * np.where(sa > 0, sa, 1.0)), 0.0)
     return out.astype(np.float32)
+
+
+# ---------------------------------------------------------------- 3DGRT primary-only tracer
+GRT_KEYS = ["rad", "dns", "hit", "hit2", "distortion", "nrm", "snrm", "hits"]
+
+
+def grt_inputs(density, sh, snormal, **kw) -> Inputs:
+    """Inputs of the 3DGRT tracer: no material, no environment light."""
+    inp = Inputs(density, np.zeros((np.asarray(density).reshape(-1, 12).shape[0], 5), np.float32), sh, snormal, None,
+                 None, max_bounces=1, enable_mis=False, **kw)
+    inp.s.material = None
+    return inp
+
+
+def grt_trace(inp: Inputs, rays: Rays, nthreads=None):
+    """3DGRT forward: rad, dns, hit (2 channels), hit2, distortion, nrm, snrm (raw weighted sum),
+    hits, visibility, margin; float64."""
+    B, H, W = rays.shape
+    out = {k: np.zeros((B, H, W, c), dtype=np.float64) for k, c in _FWD_CH.items()}
+    out["visibility"] = np.zeros(inp.s.n, dtype=np.float64)
+    out["margin"] = np.ones((B, H, W), dtype=np.float64)
+    o = OracleFwdOut(**{k: _ptr(out[k]) for k in _FWD_KEYS + ["margin"]})
+    rc = lib().oracle_grt_trace(C.byref(inp.s), C.byref(rays.r), C.byref(o), nthreads or os.cpu_count())
+    if rc != 0:
+        raise RuntimeError(f"oracle_grt_trace failed: {rc}")
+    return {k: out[k] for k in GRT_KEYS + ["visibility", "margin"]}
+
+
+def grt_trace_dual(inp: Inputs, rays: Rays, tangents: dict, nthreads=None):
+    """3DGRT forward on dual numbers: (outputs, derivatives along `tangents`: density, sh, snormal)."""
+    keep = {}
+    for name in ("density", "sh", "snormal"):
+        t = tangents.get(name)
+        arr = None if t is None else _f32(t)
+        keep[name] = arr
+        setattr(inp.s, TANGENT_FIELDS[name], _ptr(arr))
+    B, H, W = rays.shape
+    out = {k: np.zeros((B, H, W, c), dtype=np.float64) for k, c in _FWD_CH.items()}
+    der = {k: np.zeros((B, H, W, c), dtype=np.float64) for k, c in _FWD_CH.items()}
+    out["visibility"] = np.zeros(inp.s.n, dtype=np.float64)
+    out["margin"] = np.ones((B, H, W), dtype=np.float64)
+    o = OracleFwdOut(**{k: _ptr(out[k]) for k in _FWD_KEYS + ["margin"]})
+    d = OracleFwdOut(**{k: _ptr(der[k]) for k in _FWD_CH})
+    try:
+        rc = lib().oracle_grt_trace_dual(C.byref(inp.s), C.byref(rays.r), C.byref(o), C.byref(d),
+                                         nthreads or os.cpu_count())
+    finally:
+        for field in TANGENT_FIELDS.values():
+            setattr(inp.s, field, None)
+    if rc != 0:
+        raise RuntimeError(f"oracle_grt_trace_dual failed: {rc}")
+    return out, der
+
+
+def grt_trace_bwd(inp: Inputs, rays: Rays, fwd: dict, grads: dict, inclusive_end=False, nthreads=None):
+    """3DGRT backward (the reference's processHitBwd): fwd = grt_trace outputs; grads = output
+    gradients keyed rad [..,3], dns, hit [..,1], hit2, distortion, snrm [..,3] (missing = 0).
+    Returns (d_density [N,12], d_sh [N, sh_stride], d_snormal [N,3])."""
+    f = {k: np.ascontiguousarray(np.asarray(fwd[k], dtype=np.float64)) for k in
+         ("rad", "dns", "hit", "hit2", "distortion", "snrm")}
+    g = {k: _f32(grads.get(k)) for k in ("rad", "dns", "hit", "hit2", "distortion", "snrm")}
+    gi = OracleGrtIn(_ptr(f["rad"]), _ptr(f["dns"]), _ptr(f["hit"]), _ptr(f["hit2"]), _ptr(f["distortion"]),
+                     _ptr(f["snrm"]), _ptr(g["rad"]), _ptr(g["dns"]), _ptr(g["hit"]), _ptr(g["hit2"]),
+                     _ptr(g["distortion"]), _ptr(g["snrm"]), int(bool(inclusive_end)))
+    n = inp.s.n
+    dd = np.zeros((n, 12), np.float64)
+    dsh = np.zeros((n, inp.s.sh_stride), np.float64)
+    dsn = np.zeros((n, 3), np.float64)
+    go = OracleGrtOut(_ptr(dd), _ptr(dsh), _ptr(dsn))
+    rc = lib().oracle_grt_trace_bwd(C.byref(inp.s), C.byref(rays.r), C.byref(gi), C.byref(go),
+                                    nthreads or os.cpu_count())
+    if rc != 0:
+        raise RuntimeError(f"oracle_grt_trace_bwd failed: {rc}")
+    return dd, dsh, dsn

@@ -305,6 +305,20 @@ struct OracleCounters {
     int64_t paths, segments, candidates, accepted, accepted_primary, shadow_candidates, env_lookups;
     int64_t mat_atomics, env_atomics;
 };
+
+// 3DGRT primary-only backward (threedgrt_tracer/src/optixTracer.cpp:927-1012): the forward's
+// outputs [B,H,W,C] and their gradients; hit_grad has one channel (the integrated depth).
+struct OracleGrtIn {
+    const double *rad, *dns, *hit, *hit2, *distortion, *snrm;
+    const float *rad_grad, *dns_grad, *hit_grad, *hit2_grad, *distortion_grad, *snrm_grad;
+    int32_t inclusive_end;  // 1: also process the hits at exactly the last forward hit distance
+};
+
+struct OracleGrtOut {
+    double* ddensity;  // [n,12] position, density, quaternion, scale, pad
+    double* dsh;       // [n, sh_stride]
+    double* dsnormal;  // [n,3]
+};
 }
 
 namespace orc {
@@ -341,6 +355,7 @@ struct Seg {  // payLoad.h:38-100 rayPayload
     Ray<R> ray;
     Inter<R> inter;
     V3<R> radiance = v3s<R>(0), light = v3s<R>(0), contribution = v3s<R>(0), normal = v3s<R>(0);
+    V3<R> snRaw = v3s<R>(0);  // sum of weight x shading normal (the 3DGRT shading-normal output)
     R T = 1, hitDist = 0, hitDist2 = 0, lastHit = 0, distortion = 0, hits = 0;
     R scatterPdf = 0, lightPdf = 0;
     V3<R> dirGrad = v3s<R>(0);
@@ -597,6 +612,10 @@ struct Tracer {
     double* denv = nullptr;
     double* visibility = nullptr;
     const OracleFwdOut* tan_out = nullptr;  // tangents of the outputs (dual-number runs)
+    // 3DGRT primary-only tracer: the forward writes the G-buffer only, with the raw
+    // weighted sum of shading normals; grt_backward accumulates into these sinks
+    bool grt = false;
+    double *g_density = nullptr, *g_sh = nullptr, *g_snormal = nullptr;
     // decision margin of the path being traced (see OracleFwdOut::margin)
     mutable double margin = 1.0;
     void note(double a, double b) const { note_rel(&margin, a, b); }
@@ -709,9 +728,11 @@ struct Tracer {
                 const bool acc = a > 0;
                 if (acc) {
                     const Gauss<R>& G = sc.g[id];
-                    const float* mp0 = in.material + 5 * size_t(id);
-                    R mp[5];
-                    for (int q = 0; q < 5; ++q) mp[q] = input<R>(mp0[q], in.t_material, 5 * size_t(id) + q);
+                    R mp[5] = {0, 0, 0, 0, 0};  // no material tensor: the 3DGRT tracer
+                    if (in.material) {
+                        const float* mp0 = in.material + 5 * size_t(id);
+                        for (int q = 0; q < 5; ++q) mp[q] = input<R>(mp0[q], in.t_material, 5 * size_t(id) + q);
+                    }
                     const R w = a * pl.T;
                     const R proj = dot(grd, -gro);
                     const V3<R> grds = G.s * grd * proj;
@@ -748,6 +769,7 @@ struct Tracer {
         }
         const R op = R(1) - pl.T;
         pl.hitDist = depth;
+        pl.snRaw = snAcc;
         pl.valid = false;
         if (pl.hit) note(double(op), 0.5);
         if (pl.hit && op > R(0.5)) {
@@ -1429,13 +1451,14 @@ struct Tracer {
         put(&OracleFwdOut::hit2, 1, 0, c.hitDist2);
         put(&OracleFwdOut::distortion, 1, 0, c.distortion);
         if (in.enable_normals) {
+            const V3<R> sno = grt ? c.snRaw : c.inter.sn;
             put(&OracleFwdOut::nrm, 3, 0, c.normal.x); put(&OracleFwdOut::nrm, 3, 1, c.normal.y); put(&OracleFwdOut::nrm, 3, 2, c.normal.z);
-            put(&OracleFwdOut::snrm, 3, 0, c.inter.sn.x); put(&OracleFwdOut::snrm, 3, 1, c.inter.sn.y); put(&OracleFwdOut::snrm, 3, 2, c.inter.sn.z);
+            put(&OracleFwdOut::snrm, 3, 0, sno.x); put(&OracleFwdOut::snrm, 3, 1, sno.y); put(&OracleFwdOut::snrm, 3, 2, sno.z);
         }
         put(&OracleFwdOut::mat, 5, 0, c.inter.m.albedo.x); put(&OracleFwdOut::mat, 5, 1, c.inter.m.albedo.y); put(&OracleFwdOut::mat, 5, 2, c.inter.m.albedo.z);
         put(&OracleFwdOut::mat, 5, 3, c.inter.m.rough); put(&OracleFwdOut::mat, 5, 4, c.inter.m.metal);
         if (in.enable_hitcounts) put(&OracleFwdOut::hits, 1, 0, c.hits);
-        if (!p.cur.inter.valid) {
+        if (grt || !p.cur.inter.valid) {
             if (out.margin) out.margin[pix] = margin;
             return;
         }
@@ -1459,6 +1482,225 @@ struct Tracer {
     }
 
     // referenceBwdOptix.cu:27-74
+    // Real SH basis function k (0..15) at direction d, as the forward's sh_radiance weighs
+    // coefficient k (gaussianParticles.cuh:54-87).
+    static R sh_basis(int k, V3<R> d) {
+        const R C0 = 0.28209479177387814, C1 = 0.4886025119029199;
+        const R C2[] = {1.0925484305920792, -1.0925484305920792, 0.31539156525252005, -1.0925484305920792, 0.5462742152960396};
+        const R C3[] = {-0.5900435899266435, 2.890611442640554, -0.4570457994644658, 0.3731763325901154,
+                        -0.4570457994644658, 1.445305721320277, -0.5900435899266435};
+        const R x = d.x, y = d.y, z = d.z, xx = x * x, yy = y * y, zz = z * z;
+        switch (k) {
+        case 0: return C0;
+        case 1: return -C1 * y;
+        case 2: return C1 * z;
+        case 3: return -C1 * x;
+        case 4: return C2[0] * x * y;
+        case 5: return C2[1] * y * z;
+        case 6: return C2[2] * (2 * zz - xx - yy);
+        case 7: return C2[3] * x * z;
+        case 8: return C2[4] * (xx - yy);
+        case 9: return C3[0] * y * (3 * xx - yy);
+        case 10: return C3[1] * x * y * z;
+        case 11: return C3[2] * y * (4 * zz - xx - yy);
+        case 12: return C3[3] * z * (2 * zz - 3 * xx - 3 * yy);
+        case 13: return C3[4] * x * (4 * zz - xx - yy);
+        case 14: return C3[5] * z * (xx - yy);
+        default: return C3[6] * x * (xx - 3 * yy);
+        }
+    }
+
+    // d response / d grayDist times gresGrd, per kernel degree, as the 3DGRT backward writes it
+    // (gaussianParticles.cuh:202-244; degree 1 keeps the reference's sqrt factor)
+    static R response_grad(int degree, R gd, R gres, R gresGrd) {
+        switch (degree) {
+        case 8: return R(-0.000685871056241 * 4.0) * gd * gd * gd * gres * gresGrd;
+        case 5: return R(-0.0185185185185 * 2.5) * gd * std::sqrt(gd) * gres * gresGrd;
+        case 4: return R(-0.0555555555556 * 2.0) * gd * gres * gresGrd;
+        case 3: return R(-0.166666666667 * 1.5) * std::sqrt(gd) * gres * gresGrd;
+        case 1: return R(-1.5 * 0.5) * std::sqrt(gd) * gres * gresGrd;
+        case 0: return gres > 0 ? (R(0.5 * -0.329630334487) / std::sqrt(gd)) * gresGrd : R(0);
+        default: return R(-0.5) * gres * gresGrd;
+        }
+    }
+
+    // threedgrt_tracer referenceBwdOptix.cu:103-184 (raygen) and gaussianParticles.cuh:473-772
+    // (processHitBwd): the gradients of the 3DGRT G-buffer outputs with respect to every
+    // Gaussian parameter, from the forward's totals.  Candidates are re-collected on
+    // (max(0, t0 - eps), min(lastHit, t1) + eps), which in float32 excludes the hits at the
+    // last forward distance (inclusive_end keeps them).
+    void grt_backward(const OracleRays& r, int64_t pix, const OracleGrtIn& gi) {
+        const Ray<R> ray = primary_ray(r, pix);
+        cnt.paths++;
+        auto v3d = [&](const double* a) { return v3<R>(R(a[3 * pix]), R(a[3 * pix + 1]), R(a[3 * pix + 2])); };
+        auto v3f = [&](const float* a) { return a ? v3<R>(R(a[3 * pix]), R(a[3 * pix + 1]), R(a[3 * pix + 2])) : v3s<R>(0); };
+        auto f1 = [&](const float* a) { return a ? R(a[pix]) : R(0); };
+        const V3<R> radInt = v3d(gi.rad);
+        const R trmInt = R(1) - R(gi.dns[pix]);
+        const R dInt = R(gi.hit[2 * pix]), maxHit = R(gi.hit[2 * pix + 1]);
+        const R m2Int = R(gi.hit2[pix]);
+        const R distInt = R(gi.distortion[pix]);
+        const V3<R> snInt = v3d(gi.snrm);
+        const V3<R> radG = v3f(gi.rad_grad);
+        const R trmG = -f1(gi.dns_grad);
+        const R dG = f1(gi.hit_grad);
+        const R m2G = f1(gi.hit2_grad);
+        const R distG = distInt > 0 ? f1(gi.distortion_grad) : R(0);
+        const V3<R> snG = v3f(gi.snrm_grad);
+        (void)m2Int;
+        R t0, t1;
+        scene_interval(sc, ray, t0, t1);
+        R startT = std::max(R(0), back(t0));
+        const R lim = std::min(maxHit, t1);
+        const R endT = gi.inclusive_end ? R(std::nextafter(double(lim), 1e300)) : adv(lim);
+        R trm = 1, depth = 0, m2 = 0;
+        V3<R> rad = v3s<R>(0), sn = v3s<R>(0);
+        const R minT = R(in.min_transmittance);
+        bool gathered = false;
+        while (startT < endT) {
+            if (!gathered) { collect(sc, ray, cands, &margin); gathered = true; }
+            const size_t b = chunk_begin(cands, double(adv(startT)));
+            size_t e = b;
+            while (e < cands.size() && int(e - b) < in.kbuf && cands[e].t < double(endT)) ++e;
+            if (b == e) break;
+            for (size_t k = b; k < e; ++k) {
+                const uint32_t id = cands[k].id;
+                cnt.candidates++;
+                const Gauss<R>& G = sc.g[id];
+                const float* pd = in.density + 12 * size_t(id);
+                const R q[4] = {R(pd[4]), R(pd[5]), R(pd[6]), R(pd[7])};  // r, x, y, z
+                // R^T v: the reference's v * rotation with rows = columns of R
+                auto rt = [&](V3<R> v) {
+                    return v3<R>(G.rot[0][0] * v.x + G.rot[1][0] * v.y + G.rot[2][0] * v.z,
+                                 G.rot[0][1] * v.x + G.rot[1][1] * v.y + G.rot[2][1] * v.z,
+                                 G.rot[0][2] * v.x + G.rot[1][2] * v.y + G.rot[2][2] * v.z);
+                };
+                auto rv = [&](V3<R> v) {
+                    return v3<R>(G.rot[0][0] * v.x + G.rot[0][1] * v.y + G.rot[0][2] * v.z,
+                                 G.rot[1][0] * v.x + G.rot[1][1] * v.y + G.rot[1][2] * v.z,
+                                 G.rot[2][0] * v.x + G.rot[2][1] * v.y + G.rot[2][2] * v.z);
+                };
+                // d L / d q of L(v * M(q)) given g = dL/d(v * M): M[i][j] = R[j][i]
+                auto dquat = [&](V3<R> v, V3<R> g, R out[4]) {
+                    const R gv[3][3] = {{g.x * v.x, g.x * v.y, g.x * v.z},
+                                        {g.y * v.x, g.y * v.y, g.y * v.z},
+                                        {g.z * v.x, g.z * v.y, g.z * v.z}};  // [i][j] = g_i v_j
+                    const R w = q[0], x = q[1], y = q[2], z = q[3];
+                    out[0] += 2 * z * gv[0][1] - 2 * y * gv[0][2] - 2 * z * gv[1][0] + 2 * x * gv[1][2] +
+                              2 * y * gv[2][0] - 2 * x * gv[2][1];
+                    out[1] += 2 * y * gv[0][1] + 2 * z * gv[0][2] + 2 * y * gv[1][0] - 4 * x * gv[1][1] +
+                              2 * w * gv[1][2] + 2 * z * gv[2][0] - 2 * w * gv[2][1] - 4 * x * gv[2][2];
+                    out[2] += -4 * y * gv[0][0] + 2 * x * gv[0][1] - 2 * w * gv[0][2] + 2 * x * gv[1][0] +
+                              2 * z * gv[1][2] + 2 * w * gv[2][0] + 2 * z * gv[2][1] - 4 * y * gv[2][2];
+                    out[3] += -4 * z * gv[0][0] + 2 * w * gv[0][1] + 2 * x * gv[0][2] - 2 * w * gv[1][0] -
+                              4 * z * gv[1][1] + 2 * y * gv[1][2] + 2 * x * gv[2][0] + 2 * y * gv[2][1];
+                };
+                const V3<R> gposc = ray.o - G.mu;
+                const V3<R> gposcr = rt(gposc);
+                const V3<R> gro = G.is * gposcr;
+                const V3<R> rayDirR = rt(ray.d);
+                const V3<R> grdu = G.is * rayDirR;
+                const V3<R> grd = safe_normalize(grdu);
+                const V3<R> gcrod = cross(grd, gro);
+                const R grayDist = dot(gcrod, gcrod);
+                const R gres = response<R>(int(in.kernel_degree), grayDist);
+                const R galpha = std::min(R(0.99), gres * G.sigma);
+                note(double(gres), double(in.min_response));
+                note(double(gres * G.sigma), 1.0 / 255.0);
+                startT = std::max(startT, R(cands[k].t));
+                if (!(gres > R(in.min_response) && galpha > R(1.0 / 255.0))) continue;
+                const R proj = dot(grd, -gro);
+                const V3<R> grdd = grd * proj;
+                const V3<R> grds = G.s * grdd;
+                const R gsqdist = dot(grds, grds);
+                const R gdist = std::sqrt(gsqdist);
+                const R weight = galpha * trm;
+                const R nextT = (1 - galpha) * trm;
+                const bool last = nextT <= minT;
+                depth += weight * gdist;
+                const R residD = std::max(last ? R(0) : (dInt - depth) / nextT, R(0));
+                m2 += weight * gsqdist;
+                const R aHitG = (gdist - residD) * trm * dG;
+                const V3<R> grdsG = (gsqdist > 0 ? (grds * weight / gdist) * dG : v3s<R>(0)) +
+                                    (R(2) * weight * m2G) * grds + (R(2) * weight * (1 - trmInt) * distG) * grds -
+                                    (R(2) * weight * (dInt / std::max(gdist, R(1e-12))) * distG) * grds;
+                const V3<R> sclHitG = grdd * grdsG;
+                const R grdScaledDot = dot(grdsG * G.s, grd);
+                const V3<R> grdHitG = G.s * grdsG * proj - gro * grdScaledDot;
+                const V3<R> groHitG = -grd * grdScaledDot;
+                const R residTrm = galpha < R(0.999999) ? trmInt / (1 - galpha) : trm;
+                const R aDnsG = residTrm * -trmG;
+                // SH (gaussianParticles.cuh:94-170): clamped radiance, coefficient gradients
+                const float* c = in.sh + size_t(in.sh_stride) * id;
+                const int ncoef = (in.sph_degree + 1) * (in.sph_degree + 1);
+                V3<R> raw = v3s<R>(0.5);
+                for (int kk = 0; kk < ncoef; ++kk)
+                    raw += sh_basis(kk, ray.d) * v3<R>(R(c[3 * kk]), R(c[3 * kk + 1]), R(c[3 * kk + 2]));
+                const V3<R> col = v3<R>(std::max(raw.x, R(0)), std::max(raw.y, R(0)), std::max(raw.z, R(0)));
+                const V3<R> dRGB = v3<R>(raw.x > 0 ? radG.x * weight : R(0), raw.y > 0 ? radG.y * weight : R(0),
+                                         raw.z > 0 ? radG.z * weight : R(0));
+                if (g_sh)
+                    for (int kk = 0; kk < ncoef; ++kk) {
+                        const R bk = sh_basis(kk, ray.d);
+                        double* dst = g_sh + size_t(in.sh_stride) * id + 3 * kk;
+                        dst[0] += double(bk * dRGB.x);
+                        dst[1] += double(bk * dRGB.y);
+                        dst[2] += double(bk * dRGB.z);
+                    }
+                rad += weight * col;
+                const V3<R> residR = last ? v3s<R>(0) : (radInt - rad) / nextT;
+                const V3<R> residRc = v3<R>(std::max(residR.x, R(0)), std::max(residR.y, R(0)), std::max(residR.z, R(0)));
+                const float* snp = sn_of(id);
+                const V3<R> psn = snp ? v3<R>(R(snp[0]), R(snp[1]), R(snp[2])) : v3s<R>(0);
+                sn += weight * psn;
+                const V3<R> residSN = last ? v3s<R>(0) : (snInt - sn) / nextT;
+                const R aSnG = trm * dot(psn - residSN, snG);
+                if (g_snormal) {
+                    g_snormal[3 * size_t(id)] += double(weight * snG.x);
+                    g_snormal[3 * size_t(id) + 1] += double(weight * snG.y);
+                    g_snormal[3 * size_t(id) + 2] += double(weight * snG.z);
+                }
+                const R common = aHitG + aDnsG + aSnG + trm * dot(col - residRc, radG);
+                double* gd = g_density + 12 * size_t(id);
+                gd[3] += double(gres * common);
+                const R gresG = G.sigma * common;
+                const R grayDistG = response_grad(int(in.kernel_degree), grayDist, gres, gresG);
+                const V3<R> gcrodG = (R(2) * grayDistG) * gcrod;
+                // gcrod = cross(grd, gro)
+                const V3<R> grdG = v3<R>(gcrodG.z * gro.y - gcrodG.y * gro.z, gcrodG.x * gro.z - gcrodG.z * gro.x,
+                                         gcrodG.y * gro.x - gcrodG.x * gro.y);
+                const V3<R> groG = v3<R>(gcrodG.y * grd.z - gcrodG.z * grd.y, gcrodG.z * grd.x - gcrodG.x * grd.z,
+                                         gcrodG.x * grd.y - gcrodG.y * grd.x);
+                const V3<R> groT = groG + groHitG;
+                const V3<R> sclGro = v3<R>(-gposcr.x / (G.s.x * G.s.x), -gposcr.y / (G.s.y * G.s.y),
+                                           -gposcr.z / (G.s.z * G.s.z)) * groT;
+                const V3<R> gposcrG = G.is * groT;
+                const V3<R> gposcG = rv(gposcrG);
+                R dq[4] = {0, 0, 0, 0};
+                dquat(gposc, gposcrG, dq);
+                gd[0] -= double(gposcG.x);
+                gd[1] -= double(gposcG.y);
+                gd[2] -= double(gposcG.z);
+                // grd = safe_normalize(grdu)
+                const V3<R> gin = grdG + grdHitG;
+                const R l2 = dot(grdu, grdu);
+                V3<R> grduG = v3s<R>(0);
+                if (l2 > 0) {
+                    const R il = R(1) / std::sqrt(l2);
+                    grduG = gin * il - grdu * (dot(gin, grdu) * il * il * il);
+                }
+                const V3<R> sclDir = v3<R>(-rayDirR.x / (G.s.x * G.s.x), -rayDirR.y / (G.s.y * G.s.y),
+                                           -rayDirR.z / (G.s.z * G.s.z)) * grduG;
+                gd[8] += double(sclHitG.x + sclGro.x + sclDir.x);
+                gd[9] += double(sclHitG.y + sclGro.y + sclDir.y);
+                gd[10] += double(sclHitG.z + sclGro.z + sclDir.z);
+                dquat(ray.d, G.is * grduG, dq);
+                for (int qq = 0; qq < 4; ++qq) gd[4 + qq] += double(dq[qq]);
+                trm = nextT;
+            }
+        }
+    }
+
     void backward(const OracleRays& r, int64_t pix, const OracleBwdIn& bi) {
         const int64_t W = r.W, H = r.H;
         const uint32_t x = uint32_t(pix % W), y = uint32_t((pix / W) % H),
@@ -1497,7 +1739,8 @@ struct Tracer {
 
 template <class R>
 static int run(const OracleScene* in, const OracleRays* rays, const OracleFwdOut* fwd, const OracleBwdIn* bi,
-               const OracleBwdOut* bo, int nthreads, OracleCounters* counters, const OracleFwdOut* tangents = nullptr) {
+               const OracleBwdOut* bo, int nthreads, OracleCounters* counters, const OracleFwdOut* tangents = nullptr,
+               bool grt = false) {
     Scene<R> sc;
     build_scene(sc, in);
     const int64_t total = int64_t(rays->B) * rays->H * rays->W;
@@ -1515,6 +1758,7 @@ static int run(const OracleScene* in, const OracleRays* rays, const OracleFwdOut
         Tracer<R> tr(sc);
         if (fwd) tr.visibility = fwd->visibility;  // idempotent writes of 1
         tr.tan_out = tangents;
+        tr.grt = grt;
         if (bi && bo) {
             if (bo->dmat) { dm[tid].assign(size_t(n) * 5, 0.0); tr.dmat = dm[tid].data(); }
             if (bo->denv && envN) { de[tid].assign(envN, 0.0); tr.denv = de[tid].data(); }
@@ -1560,6 +1804,51 @@ static int run(const OracleScene* in, const OracleRays* rays, const OracleFwdOut
     return 0;
 }
 
+// 3DGRT backward over the launch: per-thread sinks, summed at the end.
+static int run_grt_bwd(const OracleScene* in, const OracleRays* rays, const OracleGrtIn* gi, const OracleGrtOut* go,
+                       int nthreads) {
+    Scene<double> sc;
+    build_scene(sc, in);
+    const int64_t total = int64_t(rays->B) * rays->H * rays->W;
+    const int64_t stride = std::max<int64_t>(1, rays->path_stride);
+    std::vector<int64_t> pix;
+    for (int64_t i = rays->path_offset; i < total; i += stride) pix.push_back(i);
+    nthreads = std::max(1, nthreads);
+    const size_t n = size_t(in->n);
+    std::vector<std::vector<double>> gd(nthreads), gs(nthreads), gn(nthreads);
+    std::atomic<int64_t> next{0};
+    auto worker = [&](int tid) {
+        Tracer<double> tr(sc);
+        tr.grt = true;
+        gd[tid].assign(n * 12, 0.0);
+        gs[tid].assign(n * size_t(in->sh_stride), 0.0);
+        gn[tid].assign(n * 3, 0.0);
+        tr.g_density = gd[tid].data();
+        tr.g_sh = gs[tid].data();
+        tr.g_snormal = gn[tid].data();
+        for (;;) {
+            const int64_t b = next.fetch_add(64);
+            if (b >= int64_t(pix.size())) break;
+            const int64_t e = std::min<int64_t>(b + 64, int64_t(pix.size()));
+            for (int64_t k = b; k < e; ++k) tr.grt_backward(*rays, pix[k], *gi);
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nthreads; ++t) th.emplace_back(worker, t);
+    worker(0);
+    for (auto& t : th) t.join();
+    auto sum = [&](std::vector<std::vector<double>>& parts, double* dst, size_t len) {
+        if (!dst) return;
+        std::fill(dst, dst + len, 0.0);
+        for (auto& p : parts)
+            for (size_t i = 0; i < len; ++i) dst[i] += p[i];
+    };
+    sum(gd, go->ddensity, n * 12);
+    sum(gs, go->dsh, n * size_t(in->sh_stride));
+    sum(gn, go->dsnormal, n * 3);
+    return 0;
+}
+
 }  // namespace orc
 
 extern "C" {
@@ -1585,6 +1874,28 @@ int oracle_trace_dual(const OracleScene* scene, const OracleRays* rays, const Or
                       const OracleFwdOut* tangents, int nthreads) {
     if (!scene || !rays || !out || !tangents) return 1;
     return orc::run<Dual>(scene, rays, out, nullptr, nullptr, nthreads, nullptr, tangents);
+}
+
+// 3DGRT primary-only tracer (threedgrt_tracer/src/kernels/cuda/referenceOptix.cu:105-194):
+// the G-buffer outputs of the primary segment, with the shading-normal output the raw sum of
+// weight x shading normal; the scene's material may be null.  Written into out's rad, dns,
+// hit, hit2, distortion, nrm, snrm, hits, visibility (and margin).
+int oracle_grt_trace(const OracleScene* scene, const OracleRays* rays, const OracleFwdOut* out, int nthreads) {
+    if (!scene || !rays || !out) return 1;
+    return orc::run<double>(scene, rays, out, nullptr, nullptr, nthreads, nullptr, nullptr, true);
+}
+
+int oracle_grt_trace_dual(const OracleScene* scene, const OracleRays* rays, const OracleFwdOut* out,
+                          const OracleFwdOut* tangents, int nthreads) {
+    if (!scene || !rays || !out || !tangents) return 1;
+    return orc::run<Dual>(scene, rays, out, nullptr, nullptr, nthreads, nullptr, tangents, true);
+}
+
+// 3DGRT backward (threedgrt_tracer/src/kernels/cuda/referenceBwdOptix.cu:103-184).
+int oracle_grt_trace_bwd(const OracleScene* scene, const OracleRays* rays, const OracleGrtIn* in,
+                         const OracleGrtOut* out, int nthreads) {
+    if (!scene || !rays || !in || !out || !out->ddensity) return 1;
+    return orc::run_grt_bwd(scene, rays, in, out, nthreads);
 }
 
 // Building blocks exposed for the closed-form tests.
