@@ -267,6 +267,67 @@ ptir_status ptir_trace_bwd_exact(ptir_tracer* tracer, const ptir_scene* scene, c
                                  const ptir_bwd_inputs* in, const ptir_exact_grads* extra,
                                  const ptir_bwd_outputs* out, void* stream);
 
+/* ------------------------------------------------------------------------------------
+ * 3DGRT primary-only tracer (threedgrt_tracer/, the stage-1 tracer of the same repository)
+ * on the same handle, hierarchy and k-buffer: ptir_create/ptir_build_bvh as above
+ * (ptir_options' MIS, metallic and schedule fields are ignored), then these two calls.
+ * The launch uses B, H, W, the ray tensors, frame, sph_degree and min_transmittance;
+ * render_opts must be 0 and num_images 0. */
+typedef struct ptir_grt_scene {
+    uint32_t num_gaussians;
+    const float* particle_density;   /* [N,12] pos3, density, quat(wxyz)4, scale3, pad */
+    const float* particle_radiance;  /* [N, sh_stride] SH coefficients */
+    int32_t sh_stride;
+    const float* shading_normal;     /* [N,3] (NULL = none) */
+} ptir_grt_scene;
+
+/* threedgrt OptixTracer::trace outputs (threedgrt_tracer/src/optixTracer.cpp:855-925), all
+ * [B,H,W,C] float32: the primary segment's radiance, opacity, integrated depth and last hit
+ * distance, depth second moment, distortion, normal, shading normal (the weighted sum, not
+ * normalised by the opacity, unlike ptir_trace's), hit count.  visibility as in
+ * ptir_fwd_outputs (zeroed by the caller). */
+typedef struct ptir_grt_outputs {
+    float* radiance;        /* [..,3] */
+    float* density;         /* [..,1] */
+    float* hit_distance;    /* [..,2] */
+    float* hit_distance2;   /* [..,1] */
+    float* distortion;      /* [..,1] */
+    float* normal;          /* [..,3] */
+    float* shading_normal;  /* [..,3] */
+    float* hits_count;      /* [..,1] */
+    float* visibility;      /* [N] */
+} ptir_grt_outputs;
+
+/* threedgrt OptixTracer::trace (threedgrt_tracer/src/optixTracer.cpp:855-925;
+ * kernels referenceOptix.cu:105-194). */
+ptir_status ptir_grt_trace(ptir_tracer* tracer, const ptir_grt_scene* scene, const ptir_launch* launch,
+                           const ptir_grt_outputs* out, void* stream);
+
+/* Forward outputs of ptir_grt_trace and their gradients (each gradient may be NULL = zero);
+ * hit_distance_grad is [..,1]: the integrated depth (channel 0) only. */
+typedef struct ptir_grt_bwd_inputs {
+    const float *radiance, *density, *hit_distance, *hit_distance2, *distortion, *shading_normal;
+    const float *radiance_grad, *density_grad, *hit_distance_grad, *hit_distance2_grad, *distortion_grad,
+        *shading_normal_grad;
+} ptir_grt_bwd_inputs;
+
+/* Gradients, zeroed by the call (the reference allocates them with torch::zeros):
+ * density_grad [N,12] (position, density, quaternion as given, scale), radiance_grad
+ * [N, sh_stride], shading_normal_grad [N,3] (may be NULL when the scene has no shading normal). */
+typedef struct ptir_grt_bwd_outputs {
+    float* density_grad;
+    float* radiance_grad;
+    float* shading_normal_grad;
+} ptir_grt_bwd_outputs;
+
+/* threedgrt OptixTracer::traceBwd (threedgrt_tracer/src/optixTracer.cpp:927-1012; kernels
+ * referenceBwdOptix.cu:103-184, gaussianParticles.cuh:473-772): per pixel, the forward's
+ * candidates re-collected up to its last hit distance (float32 end bound, which drops the
+ * hits at exactly that distance, as the reference does) and each hit's adjoint through the
+ * forward's totals.  The normal and hit-count outputs take no gradient. */
+ptir_status ptir_grt_trace_bwd(ptir_tracer* tracer, const ptir_grt_scene* scene, const ptir_launch* launch,
+                               const ptir_grt_bwd_inputs* in, const ptir_grt_bwd_outputs* out, void* stream);
+
 /* Bytes of a path tape for a [B,H,W] launch with max_bounces path vertices and room for
  * hits_per_path accepted hits per path (0 on invalid sizes).  No counterpart in the
  * reference: it lets ptir_trace_bwd skip re-tracing (a 64 B record per path and per

@@ -22,7 +22,8 @@ PTIR_STATUS = {
     6: "PTIR_ERR_OUT_OF_MEMORY",
 }
 
-EXPORTED = ["ptir_create", "ptir_destroy", "ptir_build_bvh", "ptir_check", "ptir_trace", "ptir_trace_bwd", "ptir_tape_bytes",
+EXPORTED = ["ptir_create", "ptir_destroy", "ptir_build_bvh", "ptir_check", "ptir_grt_trace", "ptir_grt_trace_bwd",
+            "ptir_trace", "ptir_trace_bwd", "ptir_tape_bytes",
             "ptir_last_error", "ptir_version", "ptir_cache_create", "ptir_cache_destroy", "ptir_cache_num_weights",
             "ptir_cache_table_floats", "ptir_cache_read_params", "ptir_cache_write_params", "ptir_cache_read_grads",
             "ptir_cache_query", "ptir_cache_backprop", "ptir_cache_step", "ptir_cache_train", "ptir_cache_last_error",
@@ -100,6 +101,26 @@ class Counters(C.Structure):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
+class GrtScene(C.Structure):
+    _fields_ = [("num_gaussians", C.c_uint32), ("particle_density", C.c_void_p), ("particle_radiance", C.c_void_p),
+                ("sh_stride", C.c_int32), ("shading_normal", C.c_void_p)]
+
+
+class GrtOutputs(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("radiance", "density", "hit_distance", "hit_distance2", "distortion",
+                                           "normal", "shading_normal", "hits_count", "visibility")]
+
+
+class GrtBwdInputs(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("radiance", "density", "hit_distance", "hit_distance2", "distortion",
+                                           "shading_normal", "radiance_grad", "density_grad", "hit_distance_grad",
+                                           "hit_distance2_grad", "distortion_grad", "shading_normal_grad")]
+
+
+class GrtBwdOutputs(C.Structure):
+    _fields_ = [("density_grad", C.c_void_p), ("radiance_grad", C.c_void_p), ("shading_normal_grad", C.c_void_p)]
+
+
 class CacheOptions(C.Structure):
     _fields_ = [("max_batch", C.c_uint64), ("seed", C.c_uint32), ("learning_rate", C.c_float), ("beta1", C.c_float),
                 ("beta2", C.c_float), ("epsilon", C.c_float)]
@@ -127,6 +148,10 @@ def lib() -> C.CDLL:
         L.ptir_trace_bwd_exact.argtypes = [C.c_void_p, C.POINTER(Scene), C.POINTER(Launch), C.POINTER(BwdInputs),
                                            C.POINTER(ExactGrads), C.POINTER(BwdOutputs), C.c_void_p]
         L.ptir_check.argtypes = [C.c_void_p, C.c_void_p]
+        L.ptir_grt_trace.argtypes = [C.c_void_p, C.POINTER(GrtScene), C.POINTER(Launch), C.POINTER(GrtOutputs),
+                                     C.c_void_p]
+        L.ptir_grt_trace_bwd.argtypes = [C.c_void_p, C.POINTER(GrtScene), C.POINTER(Launch), C.POINTER(GrtBwdInputs),
+                                         C.POINTER(GrtBwdOutputs), C.c_void_p]
         L.ptir_tape_bytes.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_uint32, C.c_uint32]
         L.ptir_tape_bytes.restype = C.c_uint64
         L.ptir_last_error.argtypes = [C.c_void_p]
@@ -156,7 +181,8 @@ def lib() -> C.CDLL:
         L.ptir_resolve_backward.argtypes = [C.POINTER(Resolve)] + [vp] * 10
         L.ptir_adam_step.argtypes = [C.POINTER(AdamGroup), i32, C.c_float, C.c_float, C.c_float, i32, vp, vp]
         L.ptir_ragged_layout.argtypes = [vp, i32, vp, vp, vp]
-        for name in ("ptir_create", "ptir_destroy", "ptir_build_bvh", "ptir_check", "ptir_trace", "ptir_trace_bwd", "ptir_trace_bwd_exact",
+        for name in ("ptir_create", "ptir_destroy", "ptir_build_bvh", "ptir_check", "ptir_grt_trace",
+                     "ptir_grt_trace_bwd", "ptir_trace", "ptir_trace_bwd", "ptir_trace_bwd_exact",
                      "ptir_cache_create", "ptir_cache_destroy", "ptir_cache_read_params", "ptir_cache_write_params",
                      "ptir_cache_read_grads", "ptir_cache_query", "ptir_cache_backprop", "ptir_cache_step", "ptir_cache_train",
                      "ptir_env_table_build", "ptir_env_table_build_noalloc", "ptir_camera_rays", "ptir_resolve_forward",

@@ -453,6 +453,102 @@ ptir_status ptir_ragged_layout(const int32_t* hw, int32_t n, int32_t* table, int
     return PTIR_OK;
 }
 
+// ---------------------------------------------------------------- 3DGRT primary-only tracer
+namespace {
+// The 3DGRT scene as the path tracer's: no material, no environment, no lights.
+ptir_scene grt_as_scene(const ptir_grt_scene* g) {
+    ptir_scene sc{};
+    sc.num_gaussians = g->num_gaussians;
+    sc.particle_density = g->particle_density;
+    sc.particle_material = nullptr;
+    sc.particle_radiance = g->particle_radiance;
+    sc.sh_stride = g->sh_stride;
+    sc.shading_normal = g->shading_normal;
+    sc.tab_kind = PTIR_ENV_TABLE_ALIAS;
+    return sc;
+}
+
+ptir_status check_grt(ptir_tracer* t, const ptir_grt_scene* g, const ptir_launch* la) {
+    if (!t) return fail(nullptr, PTIR_ERR_INVALID_ARGUMENT, "null tracer");
+    if (!g || !la) return fail(t, PTIR_ERR_INVALID_ARGUMENT, "null scene or launch");
+    if (!t->built) return fail(t, PTIR_ERR_NOT_BUILT, "trace called before build_bvh");
+    if (g->num_gaussians != t->bvh.n)
+        return fail(t, PTIR_ERR_NOT_BUILT, "number of Gaussians differs from the last build_bvh (" +
+                                               std::to_string(g->num_gaussians) + " vs " + std::to_string(t->bvh.n) + ")");
+    if (g->num_gaussians && (!g->particle_density || !g->particle_radiance))
+        return fail(t, PTIR_ERR_INVALID_ARGUMENT, "null particle tensor");
+    if (la->num_images != 0) return fail(t, PTIR_ERR_UNSUPPORTED, "the primary-only tracer takes [B,H,W] launches");
+    if (la->B < 0 || la->H < 0 || la->W < 0) return fail(t, PTIR_ERR_INVALID_ARGUMENT, "negative ray shape");
+    const long long paths = (long long)la->B * la->H * la->W;
+    if (paths > 0 && (!la->ray_ori || !la->ray_dir || !la->ray_to_world))
+        return fail(t, PTIR_ERR_INVALID_ARGUMENT, "null ray tensor");
+    if (paths > (1ll << 31) - 1) return fail(t, PTIR_ERR_INVALID_ARGUMENT, "more than 2^31-1 paths in one launch");
+    if (la->render_opts != 0u) return fail(t, PTIR_ERR_INVALID_ARGUMENT, "render_opts must be 0");
+    if (la->sph_degree < 0 || la->sph_degree > 3) return fail(t, PTIR_ERR_INVALID_ARGUMENT, "sph_degree must be in [0,3]");
+    if (g->sh_stride < 3 * (la->sph_degree + 1) * (la->sph_degree + 1))
+        return fail(t, PTIR_ERR_INVALID_ARGUMENT, "particle_radiance has fewer coefficients than sph_degree needs");
+    return PTIR_OK;
+}
+}  // namespace
+
+ptir_status ptir_grt_trace(ptir_tracer* t, const ptir_grt_scene* g, const ptir_launch* la, const ptir_grt_outputs* out,
+                           void* stream) {
+    ptir_status st = check_grt(t, g, la);
+    if (st != PTIR_OK) return st;
+    if (!out || !out->radiance || !out->density || !out->hit_distance || !out->hit_distance2 || !out->distortion ||
+        !out->normal || !out->shading_normal || !out->hits_count || (g->num_gaussians && !out->visibility))
+        return fail(t, PTIR_ERR_INVALID_ARGUMENT, "null output tensor");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipError_t e = ptir::prep_records(t->bvh, kopts(t->opts), g->particle_density, nullptr, g->shading_normal, s);
+    if (e != hipSuccess) return hip_fail(t, e, "prep_records");
+    const ptir_scene sc = grt_as_scene(g);
+    ptir::DeviceScene S = make_scene(t, &sc, la);
+    S.max_bounces = 1u;
+    const ptir::grt::FwdOut o{out->radiance, out->density, out->hit_distance, out->hit_distance2, out->distortion,
+                              out->normal, out->shading_normal, out->hits_count,
+                              reinterpret_cast<uint32_t*>(out->visibility)};
+    e = ptir::launch_grt_fwd(S, o, t->opts.kbuffer, t->short_stack, s);
+    if (e != hipSuccess) return hip_fail(t, e, "primary-only trace launch");
+    return PTIR_OK;
+}
+
+ptir_status ptir_grt_trace_bwd(ptir_tracer* t, const ptir_grt_scene* g, const ptir_launch* la,
+                               const ptir_grt_bwd_inputs* in, const ptir_grt_bwd_outputs* out, void* stream) {
+    ptir_status st = check_grt(t, g, la);
+    if (st != PTIR_OK) return st;
+    if (!in || !out) return fail(t, PTIR_ERR_INVALID_ARGUMENT, "null backward inputs or outputs");
+    const long long paths = (long long)la->B * la->H * la->W;
+    if (paths > 0 && (!in->radiance || !in->density || !in->hit_distance || !in->hit_distance2 || !in->distortion ||
+                      !in->shading_normal))
+        return fail(t, PTIR_ERR_INVALID_ARGUMENT, "null forward output");
+    const size_t N = g->num_gaussians;
+    if (N && (!out->density_grad || !out->radiance_grad || (g->shading_normal && !out->shading_normal_grad)))
+        return fail(t, PTIR_ERR_INVALID_ARGUMENT, "null gradient output");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipError_t e = hipSuccess;
+    // the reference allocates every gradient with torch::zeros (threedgrt optixTracer.cpp:953-955)
+    if (N) {
+        e = ptir::fill_zero(out->density_grad, N * 12 * sizeof(float), s);
+        if (e == hipSuccess) e = ptir::fill_zero(out->radiance_grad, N * size_t(g->sh_stride) * sizeof(float), s);
+        if (e == hipSuccess && out->shading_normal_grad)
+            e = ptir::fill_zero(out->shading_normal_grad, N * 3 * sizeof(float), s);
+        if (e != hipSuccess) return hip_fail(t, e, "zeroing gradients");
+    }
+    e = ptir::prep_records(t->bvh, kopts(t->opts), g->particle_density, nullptr, g->shading_normal, s);
+    if (e != hipSuccess) return hip_fail(t, e, "prep_records");
+    const ptir_scene sc = grt_as_scene(g);
+    ptir::DeviceScene S = make_scene(t, &sc, la);
+    S.max_bounces = 1u;
+    const ptir::grt::BwdIO io{in->radiance, in->density, in->hit_distance, in->hit_distance2, in->distortion,
+                              in->shading_normal, in->radiance_grad, in->density_grad, in->hit_distance_grad,
+                              in->hit_distance2_grad, in->distortion_grad, in->shading_normal_grad,
+                              g->particle_density, g->shading_normal, out->density_grad, out->radiance_grad,
+                              g->shading_normal ? out->shading_normal_grad : nullptr};
+    e = ptir::launch_grt_bwd(S, io, t->opts.kbuffer, t->short_stack, s);
+    if (e != hipSuccess) return hip_fail(t, e, "primary-only backward launch");
+    return PTIR_OK;
+}
+
 uint64_t ptir_tape_bytes(int32_t B, int32_t H, int32_t W, uint32_t max_bounces, uint32_t hits_per_path) {
     if (B < 0 || H < 0 || W < 0 || hits_per_path == 0 || hits_per_path > 0xFFFFu || max_bounces > 64u) return 0;
     return ptir::tape_bytes((uint64_t)B * H * W, max_bounces, hits_per_path);

@@ -285,11 +285,12 @@ __global__ void __launch_bounds__(256) k_records(const float* __restrict__ densi
     grec[4 * j + 1] = make_float4(m[0][1] * is[1], m[1][1] * is[1], m[2][1] * is[1], p[1]);
     grec[4 * j + 2] = make_float4(m[0][2] * is[2], m[1][2] * is[2], m[2][2] * is[2], p[2]);
     grec[4 * j + 3] = make_float4(p[3], kernel_scale(p[3], o), __int_as_float(int(i)), 0.f);
-    const float* mt = material + 5 * size_t(i);
+    // material may be null (the primary-only tracer has none)
+    const float* mt = material ? material + 5 * size_t(i) : nullptr;
     const float sn[3] = {snormal ? snormal[3 * i] : 0.f, snormal ? snormal[3 * i + 1] : 0.f,
                          snormal ? snormal[3 * i + 2] : 0.f};
-    srec[2 * j + 0] = make_float4(mt[0], mt[1], mt[2], mt[3]);
-    srec[2 * j + 1] = make_float4(mt[4], sn[0], sn[1], sn[2]);
+    srec[2 * j + 0] = mt ? make_float4(mt[0], mt[1], mt[2], mt[3]) : make_float4(0.f, 0.f, 0.f, 0.f);
+    srec[2 * j + 1] = make_float4(mt ? mt[4] : 0.f, sn[0], sn[1], sn[2]);
     prec[j] = make_float4(m[0][0] * s[0] + m[0][1] * s[1] + m[0][2] * s[2],
                           m[1][0] * s[0] + m[1][1] * s[1] + m[1][2] * s[2],
                           m[2][0] * s[0] + m[2][1] * s[1] + m[2][2] * s[2], snormal ? 1.f : 0.f);

@@ -204,6 +204,22 @@ struct AdjIO {
     float4* scratch;
 };
 
+// 3DGRT primary-only tracer (grt_path.hpp): forward outputs, and the backward's forward
+// outputs, output gradients, raw Gaussian parameters and gradient sinks.
+namespace grt {
+struct FwdOut {
+    float *rad, *dns, *hit, *hit2, *dist, *nrm, *snrm, *hits;  // [P,3], [P], [P,2], [P], [P], [P,3], [P,3], [P]
+    uint32_t* vis;                                              // [N]
+};
+struct BwdIO {
+    const float *rad, *dns, *hit, *hit2, *dist, *snrm;                               // forward outputs
+    const float *rad_grad, *dns_grad, *hit_grad, *hit2_grad, *dist_grad, *snrm_grad;  // hit_grad: [P] (depth)
+    const float* density;  // [N,12] position, density, quaternion (w,x,y,z), scale, pad
+    const float* snormal;  // [N,3] or null
+    float *d_density, *d_sh, *d_snormal;  // [N,12], [N, sh_stride], [N,3] (zeroed by the caller)
+};
+}  // namespace grt
+
 enum CounterId { C_PATHS = 0, C_SEGMENTS, C_CANDIDATES, C_ACCEPTED, C_ACCEPTED_PRIMARY, C_SHADOW_CANDIDATES,
                  C_ENV_LOOKUPS, C_MAT_ATOMICS, C_ENV_ATOMICS, C_TRAVERSALS, C_NODES, C_LEAF_TESTS,
                  C_COUNT };

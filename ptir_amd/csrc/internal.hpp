@@ -102,6 +102,9 @@ hipError_t adam_step(const AdamGroup* groups, int n, float b1, float b2, float e
 hipError_t resolve_backward(const ResolveIO& io, const ResolveGrads& g, hipStream_t s);
 
 hipError_t launch_trace_fwd(const DeviceScene& S, const FwdOut& out, int kbuf, bool short_stack, hipStream_t s);
+// 3DGRT primary-only tracer (grt.hip)
+hipError_t launch_grt_fwd(const DeviceScene& S, const grt::FwdOut& out, int kbuf, bool short_stack, hipStream_t s);
+hipError_t launch_grt_bwd(const DeviceScene& S, const grt::BwdIO& io, int kbuf, bool short_stack, hipStream_t s);
 hipError_t launch_trace_bwd(const DeviceScene& S, const BwdIO& io, int kbuf, bool short_stack, hipStream_t s);
 // Wavefront schedule of the forward (wavefront.hip); `work` holds wavefront_bytes(paths, max_bounces).
 uint64_t wavefront_bytes(uint64_t paths, uint32_t max_bounces);
