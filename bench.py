@@ -90,6 +90,47 @@ def cpu_baseline(scene, table, stride: int) -> dict:
                       f"incl. the oracle's own BVH build, {dt:.1f} s"}
 
 
+def time_primary_only(steps: int = 10, warmup: int = 2) -> dict:
+    """The 3DGRT primary-only tracer on the object-scale rays: G-buffer forward and the
+    backward to every Gaussian parameter (random output gradients), timed with HIP events."""
+    import torch
+
+    from ptir_amd import scenes
+    from ptir_amd.grt import GRTTracer
+
+    dev = torch.device("cuda:0")
+    sc = scenes.object_scene(0)
+    g = sc.to(dev)
+    out = {"gaussians": sc.num_gaussians, "rays": sc.num_paths}
+    for builder in ("lbvh", "ploc"):
+        tr = GRTTracer(builder=builder)
+        args = (0, g.ray_to_world, g.rays_ori, g.rays_dir, g.particle_density(), g.features, g.shading_normal)
+        gen = torch.Generator(device=dev).manual_seed(0)
+        tr.build_bvh(g.positions, g.rotation, g.scale, g.density)
+        outs = tr.trace(*args, 0, sc.sph_degree, sc.min_transmittance)
+        grads = [torch.randn(o.shape, device=dev, generator=gen) for o in outs[:7]]
+        grads[2] = grads[2][..., :1].contiguous()
+        ev = []
+        for i in range(warmup + steps):
+            e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+            tr.build_bvh(g.positions, g.rotation, g.scale, g.density)
+            e0.record()
+            outs = tr.trace(*args, 0, sc.sph_degree, sc.min_transmittance)
+            e1.record()
+            tr.trace_bwd(*args[:4], *outs[:7], *args[4:], *grads, 0, sc.sph_degree, sc.min_transmittance)
+            e2.record()
+            if i >= warmup:
+                ev.append((e0, e1, e2))
+        torch.cuda.synchronize()
+        fwd = sum(a.elapsed_time(b) for a, b, _ in ev) / steps
+        bwd = sum(b.elapsed_time(c) for _, b, c in ev) / steps
+        out[builder] = {"ms_fwd": round(fwd, 3), "ms_bwd": round(bwd, 3),
+                        "rays_per_s_fwd_bwd": round(sc.num_paths / ((fwd + bwd) * 1e-3), 1)}
+        del tr, outs, grads
+    torch.cuda.empty_cache()
+    return out
+
+
 def nrc_comparison(train_steps: int = 150, timed_steps: int = 5, spps=(1, 4, 16, 64)) -> dict:
     """Radiance cache (NRC) form against the plain tracer on the object-scale workload.
 
@@ -372,6 +413,9 @@ def main() -> int:
         "roofline": roofline,
         "work_units": {"fwd": cf, "bwd": cb},
     }
+    if not args.no_workloads:
+        print("primary-only tracer", file=sys.stderr, flush=True)
+        line["primary_only"] = time_primary_only()
     if not args.no_nrc:
         del tape, tk
         tape, tk = None, {}

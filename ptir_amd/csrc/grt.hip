@@ -20,12 +20,16 @@ k_grt_fwd(DeviceScene S, grt::FwdOut out, const float4* __restrict__ nodes, cons
 
 template <int K, int SD>
 __global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(waves_per_eu(K, SD))))
-k_grt_bwd(DeviceScene S, grt::BwdIO io) {
+k_grt_bwd(DeviceScene S, grt::BwdIO io, const float4* __restrict__ nodes, const float4* __restrict__ grec) {
+    S.nodes = nodes;
+    S.grec = grec;
     __shared__ uint32_t lds[LdsLayout<K, SD>::kWords * kTraceBlock];
+    __shared__ uint32_t wst[(kTraceBlock / 64) * kStackDeep];
     const Lane L{lds + threadIdx.x};
+    uint32_t* wstack = wst + (threadIdx.x / 64) * kStackDeep;
     const PixelId px = pixel_of(S);
     if (!px.ok) return;
-    grt::bwd_path<K, SD>(S, io, L, px.pix);
+    grt::bwd_path<K, SD>(S, io, L, S.wave_traversal ? wstack : nullptr, px.pix);
 }
 
 namespace {
@@ -44,7 +48,7 @@ struct GrtBwdLaunch {
     hipStream_t s;
     template <int K, int SD>
     void run() const {
-        k_grt_bwd<K, SD><<<S.num_blocks, kTraceBlock, 0, s>>>(S, io);
+        k_grt_bwd<K, SD><<<S.num_blocks, kTraceBlock, 0, s>>>(S, io, S.nodes, S.grec);
     }
 };
 }  // namespace

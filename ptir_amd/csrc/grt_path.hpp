@@ -267,9 +267,12 @@ __device__ __forceinline__ void hit_bwd(const DeviceScene& S, const BwdIO& io, c
 }
 
 // threedgrt_tracer/src/kernels/cuda/referenceBwdOptix.cu:103-184 for one pixel: the forward's
-// candidates re-collected on (max(0, t0 - eps), min(lastHit, t1) + eps), in order.
+// candidates re-collected on (max(0, t0 - eps), min(lastHit, t1) + eps), in order.  wstack:
+// the wave's node stack for the wave-coherent traversal (every lane of the wave calls
+// together), or null.
 template <int K, int SD>
-__device__ __forceinline__ void bwd_path(const DeviceScene& S, const BwdIO& io, const Lane& L, size_t pix) {
+__device__ __forceinline__ void bwd_path(const DeviceScene& S, const BwdIO& io, const Lane& L, uint32_t* wstack,
+                                         size_t pix) {
     const Ray ray = primary_ray(S, pix);
     auto v3at = [&](const float* a) { return a ? v3(a[3 * pix], a[3 * pix + 1], a[3 * pix + 2]) : v3s(0.f); };
     BwdPixel b;
@@ -293,6 +296,23 @@ __device__ __forceinline__ void bwd_path(const DeviceScene& S, const BwdIO& io, 
     float startT = fmaxf(0.f, t0 - kEpsT);
     const float endT = fminf(maxHit, t1) + kEpsT;
     count(S, C_PATHS);
+    if (wstack) {
+        bool done = false;
+        for (;;) {
+            const bool need = !done && startT < endT;
+            if (__ballot(need) == 0) break;
+            const int cnt = wave_collect<K, SD>(S, ray, startT + kEpsT, endT, need, L, wstack);
+            if (need) {
+                if (cnt == 0) done = true;
+                for (int k = 0; k < cnt; ++k) {
+                    const float t = __uint_as_float(L.at(k));
+                    hit_bwd(S, io, ray, int(L.at(K + k)), b);
+                    startT = fmaxf(startT, t);
+                }
+            }
+        }
+        return;
+    }
     while (startT < endT) {
         const int cnt = collect<K, SD>(S, ray, startT + kEpsT, endT, L);
         if (cnt == 0) break;

@@ -21,10 +21,12 @@ int main(int argc, char** argv) {
     std::vector<uint32_t> vis(size_t(N), 0u);
     const grt::FwdOut fo{rad.data(), dns.data(), hit.data(), hit2.data(), dist.data(), nrm.data(), snrm.data(),
                          hits.data(), vis.data()};
+    la.S.wave_traversal = int(la.m.get("wave_traversal", 0));
     std::vector<uint32_t> lds(size_t(LdsLayout<16, kStackDeep>::kWords) * kTraceBlock, 0u);
+    std::vector<uint32_t> wst(kStackDeep, 0u);  // one-lane waves: one wave stack
     la.for_each_path([&](const PixelId& px, int t) {
         const Lane L{lds.data() + t};
-        grt::fwd_path<16, kStackDeep>(la.S, fo, L, nullptr, px.pix);
+        grt::fwd_path<16, kStackDeep>(la.S, fo, L, la.S.wave_traversal ? wst.data() : nullptr, px.pix);
     });
     host::write_f32("f_rad", rad);
     host::write_f32("f_dns", dns);
@@ -47,7 +49,7 @@ int main(int argc, char** argv) {
                         d_sh.data(),  d_snormal.data()};
     la.for_each_path([&](const PixelId& px, int t) {
         const Lane L{lds.data() + t};
-        grt::bwd_path<16, kStackDeep>(la.S, io, L, px.pix);
+        grt::bwd_path<16, kStackDeep>(la.S, io, L, la.S.wave_traversal ? wst.data() : nullptr, px.pix);
     });
     host::write_f32("d_density", d_density);
     host::write_f32("d_sh", d_sh);

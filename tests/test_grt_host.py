@@ -28,8 +28,8 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
 
 
-@pytest.mark.parametrize("seed", [3, 8])
-def test_grt_host_build_matches_oracle(harness, tmp_path, seed):
+@pytest.mark.parametrize("seed,wave", [(3, 0), (8, 1)])
+def test_grt_host_build_matches_oracle(harness, tmp_path, seed, wave):
     sc = scenes.small_scene(seed, n=1024, res=24, spp=1, max_bounces=1)
     sc.density = torch.clamp(sc.density, max=0.9)
     inp = O.grt_inputs(sc.particle_density(), sc.features, sc.shading_normal, sph_degree=sc.sph_degree,
@@ -46,7 +46,7 @@ def test_grt_host_build_matches_oracle(harness, tmp_path, seed):
          "distortion": rng.standard_normal((B, H, W, 1)) * k, "snrm": rng.standard_normal((B, H, W, 3)) * k}
     g = {key: v.astype(np.float32) for key, v in g.items()}
     wd = str(tmp_path)
-    E.write_launch(wd, sc, env_table(sc), max_bounces=1)
+    E.write_launch(wd, sc, env_table(sc), max_bounces=1, wave_traversal=wave)
     for key, name in (("rad", "g_rad"), ("dns", "g_dns"), ("hit", "g_hit"), ("hit2", "g_hit2"),
                       ("distortion", "g_dist"), ("snrm", "g_snrm")):
         E.write_f32(wd, name, g[key])
