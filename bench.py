@@ -299,7 +299,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--kbuffer", type=int, default=0, help="k-buffer depth (0: the autotuner's choice)")
     ap.add_argument("--no-autotune", action="store_true",
-                    help="use k-buffer 16, per-wave primary traversal and the one-lane kernel instead of tuning")
+                    help="use k-buffer 16, per-wave primary traversal, the one-lane kernel and PLOC instead of tuning")
     ap.add_argument("--cpu-stride", type=int, default=8, help="oracle path sample stride for cpu_baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tape", action="store_true", help="backward re-traces every segment (reference behaviour)")
@@ -344,7 +344,7 @@ def main() -> int:
     kw = dict(render_opts=0, sph_degree=scene.sph_degree, min_transmittance=scene.min_transmittance,
               max_bounces=scene.max_bounces)
     tuned = None
-    tcfg = dict(kbuffer=16, traversal="wave", schedule="megakernel", builder="lbvh")
+    tcfg = dict(kbuffer=16, traversal="wave", schedule="megakernel", builder="ploc")
     if not args.no_autotune:
         print("autotuning the object-scale step", file=sys.stderr, flush=True)
         tuned = autotune((g.positions, g.rotation, g.scale, g.density), fargs[4:], scene.frame, g.ray_to_world,
