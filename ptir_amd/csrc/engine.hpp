@@ -1,0 +1,238 @@
+// Traversal engine for incoherent rays: persistent waves whose lanes each walk one ray and, when
+// the walk is over, take the next ray from a queue (dynamic fetch, after Aila and Laine,
+// "Understanding the Efficiency of Ray Traversal on GPUs", HPG 2009).  In a wave that walks a
+// fixed set of 64 rays, the wave runs until its longest walk ends; with shadow and bounce rays
+// that kept about one lane in ten busy.  Here a lane whose walk ended gets a new ray at the next
+// phase boundary.
+//
+// A walk follows collect()'s rules (trace_path.hpp) exactly -- the K nearest candidates of the
+// ray in (tmin, tmax), sorted by (t, leaf slot), node steps and leaf tests in separate phases
+// with a queue of four leaves per lane -- so every ray gets the candidates collect() returns.
+#pragma once
+
+#include "trace_path.hpp"
+
+namespace ptir {
+namespace eng {
+
+template <int K, int SD>
+struct Walk {
+    float bt[K];
+    uint32_t bs[K];
+    V3 inv;
+    float tmin, tmax;
+    int link, sp, q0, q1, q2, q3;
+    bool walk;
+
+    __device__ __forceinline__ void idle() {
+        walk = false;
+        q0 = q1 = q2 = q3 = -1;
+    }
+
+    // A new walk of ray r in (lo, hi).
+    __device__ __forceinline__ void start(const DeviceScene& S, const Ray& r, float lo, float hi) {
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            bt[j] = __int_as_float(0x7f800000);
+            bs[j] = kInvalid;
+        }
+        count(S, C_TRAVERSALS);
+        inv = v3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
+        tmin = lo;
+        tmax = hi;
+        q0 = q1 = q2 = q3 = -1;
+        sp = 0;
+        link = S.root;
+        walk = S.n > 0 && S.root >= 0;
+        if (S.n > 0 && S.root < 0) q0 = ~S.root;  // one Gaussian: its leaf only
+    }
+
+    __device__ __forceinline__ bool done() const { return !walk && q0 < 0; }
+
+    __device__ __forceinline__ void push(int s) {
+        q3 = q2;
+        q2 = q1;
+        q1 = q0;
+        q0 = s;
+    }
+
+    __device__ __forceinline__ void insert(float t, uint32_t s) {
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const bool lt = (t < bt[j]) || (t == bt[j] && s < bs[j]);
+            const float tt = lt ? bt[j] : t;
+            const uint32_t ss = lt ? bs[j] : s;
+            bt[j] = lt ? t : bt[j];
+            bs[j] = lt ? s : bs[j];
+            t = tt;
+            s = ss;
+        }
+    }
+
+    // One node phase and one leaf phase; every active lane of the wave calls together (a lane
+    // with no walk stands by).
+    __device__ __forceinline__ void burst(const DeviceScene& S, const Ray& r, const Lane& L) {
+        for (;;) {
+            if (__ballot(walk && q0 < 0) == 0) break;  // every walking lane has a leaf queued
+            if (walk && q2 < 0) {                         // room for both children
+                const float4* nd = S.nodes + 4 * link;
+                const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2], n3 = nd[3];
+                count(S, C_NODES);
+                const float cut = fminf(tmax, bt[K - 1]);
+                float tl, tr;
+                bool hl = slab(n0, n1, r, inv, tmin, cut, tl);
+                bool hr = slab(n2, n3, r, inv, tmin, cut, tr);
+                const int cl = __float_as_int(n0.w), cr = __float_as_int(n1.w);
+                if (hl && cl < 0) { push(~cl); hl = false; }
+                if (hr && cr < 0) { push(~cr); hr = false; }
+                if (hl && hr) {
+                    const bool lf = tl <= tr;
+                    if (sp >= SD) {  // cannot happen after the build-time depth check
+                        atomicOr(S.error_flag, 1);
+                        walk = false;
+                    } else {
+                        L.at(sp++) = uint32_t(lf ? cr : cl);
+                        link = lf ? cl : cr;
+                    }
+                } else if (hl) {
+                    link = cl;
+                } else if (hr) {
+                    link = cr;
+                } else if (sp == 0) {
+                    walk = false;
+                } else {
+                    link = int(L.at(--sp));
+                }
+            }
+        }
+        while (__ballot(q0 >= 0) != 0) {
+            if (q0 >= 0) {
+                const int slot = q0;
+                q0 = q1;
+                q1 = q2;
+                q2 = q3;
+                q3 = -1;
+                float t;
+                count(S, C_LEAF_TESTS);
+                if (cand_test(S, slot, r, tmin, tmax, t) &&
+                    (t < bt[K - 1] || (t == bt[K - 1] && uint32_t(slot) < bs[K - 1])))
+                    insert(t, uint32_t(slot));
+            }
+        }
+    }
+};
+
+// The lanes of a wave that want a job take consecutive indices from *head (one atomic per
+// wave); returns this lane's index (meaningful when want).
+__device__ __forceinline__ uint32_t fetch(uint32_t* head, bool want) {
+    const uint64_t m = __ballot(want);
+    if (m == 0) return 0u;
+    const int leader = __ffsll((long long)m) - 1;
+    const int lane = int(__lane_id());
+    uint32_t base = 0u;
+    if (lane == leader) base = atomicAdd(head, uint32_t(__popcll(m)));
+    base = __shfl(base, leader);
+    return base + uint32_t(__popcll(m & ((1ull << lane) - 1ull)));
+}
+
+constexpr int kRefillLanes = 16;
+
+// A shadow ray job: the NEE sample of a path and the ray to test (16 words).  The engine writes
+// back the random stream after the test and whether the ray is occluded.
+struct ShadowJob {
+    float4 a;  // origin, direction.x
+    float4 b;  // direction.y, direction.z, random stream (low, high)
+    float4 c;  // NEE contribution (before MIS), environment pdf
+    float4 d;  // scatter pdf, the path's slot in the NEE stage, occluded (0/1), unused
+};
+
+// pathTracer.cuh:161-214 (the stochastic shadow ray of shadow_occluded) over a queue of jobs:
+// every job gets the candidates, draws and result shadow_occluded would give it.  Every lane of
+// the wave runs it; returns when the queue is exhausted and every lane's job is done.
+template <int K, int SD>
+__device__ __forceinline__ void occlusion(const DeviceScene& S, ShadowJob* jobs, uint32_t njobs, uint32_t* head,
+                                          const Lane& L) {
+    Walk<K, SD> w;
+    w.idle();
+    bool has = false;
+    uint32_t j = 0u;
+    Ray ray{v3s(0.f), v3s(0.f)};
+    Rng rng;
+    rng.state = 0ull;
+    rng.inc = 1ull;
+    float startT = 0.f, endT = 0.f;
+    auto finish = [&](bool occ) {
+        ShadowJob* q = jobs + j;
+        q->b.z = __uint_as_float(uint32_t(rng.state));
+        q->b.w = __uint_as_float(uint32_t(rng.state >> 32));
+        q->d.z = occ ? 1.f : 0.f;
+        has = false;
+    };
+    const uint64_t lanes = __ballot(1);
+    bool dry = false;  // the queue is exhausted (wave-uniform, sticky)
+    for (;;) {
+        // refill once a quarter of the wave is idle (or all of it): one atomic on the shared
+        // queue head per refill, not per finished ray
+        const uint64_t idle = __ballot(!has);
+        const bool refill = !dry && (__popcll(idle) >= kRefillLanes || idle == lanes);
+        const bool want = refill && !has;
+        const uint32_t idx = fetch(head, want);
+        if (refill) dry = __ballot(want && idx >= njobs) != 0;
+        if (want && idx < njobs) {
+            j = idx;
+            const ShadowJob q = jobs[j];
+            ray.o = v3(q.a.x, q.a.y, q.a.z);
+            ray.d = v3(q.a.w, q.b.x, q.b.y);
+            rng.state = uint64_t(__float_as_uint(q.b.z)) | (uint64_t(__float_as_uint(q.b.w)) << 32);
+            float t0, t1;
+            scene_interval(S, ray, t0, t1);
+            startT = fmaxf(0.f, t0 - kEpsT);
+            endT = fminf(kInfDist, t1) + kEpsT;
+            has = true;
+            if (startT < endT) w.start(S, ray, startT + kEpsT, endT);
+            else finish(false);
+        }
+        if (__ballot(has) == 0) {
+            if (dry) break;
+            continue;  // every new job ended at once: take more
+        }
+        w.burst(S, ray, L);
+        if (has && w.done()) {
+            // the chunk, sorted, through the lane's LDS words (its stack is free now), then
+            // shadow_occluded's loop over it
+            int cnt = 0;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                L.at(k) = __float_as_uint(w.bt[k]);
+                L.at(K + k) = w.bs[k];
+                cnt += w.bs[k] != kInvalid ? 1 : 0;
+            }
+            float batchEnd = startT;
+            bool occ = false;
+            for (int k = 0; k < cnt; ++k) {
+                const float t = __uint_as_float(L.at(k));
+                const int slot = int(L.at(K + k));
+                if (t > endT) break;
+                batchEnd = fmaxf(batchEnd, t);
+                count(S, C_SHADOW_CANDIDATES);
+                if (self_occluded(S, slot, t, ray.d)) continue;
+                HitGeom h;
+                const float a = hit_alpha(S, slot, ray, h);
+                if (a > 0.f && rng.next_1d() < a) {
+                    occ = true;
+                    break;
+                }
+            }
+            if (occ || cnt == 0 || batchEnd <= startT) {
+                finish(occ);
+            } else {
+                startT = batchEnd;
+                if (startT < endT) w.start(S, ray, startT + kEpsT, endT);
+                else finish(false);
+            }
+        }
+    }
+}
+
+}  // namespace eng
+}  // namespace ptir

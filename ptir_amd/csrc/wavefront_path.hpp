@@ -7,8 +7,11 @@
 //   k_wf_primary  the primary segment of every path, on the pixel tiles of k_trace_fwd (the
 //                 rays of a wave walk the hierarchy together); paths whose first vertex is an
 //                 interaction go on to a queue;
-//   k_wf_nee      next-event estimation (environment sample and stochastic shadow ray) of the
-//                 queued paths (with MIS on);
+//   k_wf_nee_setup, k_wf_occlusion, k_wf_nee_finish
+//                 next-event estimation of the queued paths (with MIS on): the environment
+//                 sample and the shadow ray to test (a job per path that has one), the
+//                 stochastic shadow rays on the traversal engine of engine.hpp (lanes take a
+//                 new ray when theirs is done), then the NEE record of each path;
 //   k_wf_bounce   one launch per further path vertex: BRDF sample, Russian roulette, the next
 //                 segment and its contribution; paths that go on are queued for the next launch.
 // The NEE and bounce kernels are persistent: each wave takes the next 64 queue entries until
@@ -20,7 +23,7 @@
 // outputs, tape and cache records.
 #pragma once
 
-#include "trace_path.hpp"
+#include "engine.hpp"
 
 namespace ptir {
 namespace wf {
@@ -56,6 +59,8 @@ struct IO {
     uint32_t* qa;     // path indices of the stages with even index
     uint32_t* qb;     // and with odd index
     uint32_t* ctrl;   // per stage s: ctrl[2s] = entries pushed, ctrl[2s + 1] = entries taken
+    uint32_t* jctl;   // shadow jobs: [0] pushed, [1] taken by the engine, [2] taken by the finish
+    eng::ShadowJob* jobs;  // [P]
     size_t P;
     __device__ __forceinline__ uint32_t* queue(int s) const { return (s & 1) ? qb : qa; }
     __device__ __forceinline__ float* state(int s) const { return (s & 1) ? sb : sa; }
@@ -170,12 +175,13 @@ __device__ __forceinline__ uint32_t push(const IO& io, int s, bool want, uint32_
     return slot;
 }
 
-// The next 64 entries of stage s for this wave (first index; >= count when exhausted).
-__device__ __forceinline__ uint32_t take(const IO& io, int s) {
+// The next 64 entries of a queue for this wave (first index; >= count when exhausted).
+__device__ __forceinline__ uint32_t take_at(uint32_t* taken) {
     uint32_t base = 0u;
-    if (__lane_id() == 0u) base = atomicAdd(io.ctrl + 2 * s + 1, 64u);
+    if (__lane_id() == 0u) base = atomicAdd(taken, 64u);
     return __shfl(base, 0);
 }
+__device__ __forceinline__ uint32_t take(const IO& io, int s) { return take_at(io.ctrl + 2 * s + 1); }
 
 __device__ __forceinline__ void tape_resume(SegTape& tp, const FwdOut& out, size_t pix, size_t paths, const Carry& c) {
     path_tape_begin(tp, out.tape, pix, paths);
@@ -249,12 +255,47 @@ __device__ __forceinline__ bool primary_path(const DeviceScene& S, const FwdOut&
     return go;
 }
 
-// Next-event estimation of a queued path, state in p, rng, c (sample_nee and the NEE record of
-// the tape).  Returns whether the path goes on to stage 1.
-template <int K, int SD>
-__device__ __forceinline__ bool nee_path(const DeviceScene& S, const FwdOut& out, const Lane& L, uint32_t pix, Path& p,
-                                         Rng& rng, Carry& c) {
-    sample_nee<K, SD>(S, p, rng, L);
+// sample_nee up to its shadow ray, for a queued path with state in p, rng: returns whether the
+// path has a shadow ray to test, and then fills the job (the ray, the random stream, the
+// contribution and pdfs the NEE record takes if the ray is not occluded).
+__device__ __forceinline__ bool nee_setup(const DeviceScene& S, Path& p, Rng& rng, uint32_t slot, eng::ShadowJob& job) {
+    emit_clear(p);
+    if (!has_env_light(S)) return false;
+    float envPdf;
+    const V3 wi = sample_env(S, rng, envPdf);
+    const V3 Li = background(S, wi);
+    if (envPdf <= 0.f || is_zero(Li)) return false;
+    if (dot(p.cur.sn, wi) <= 0.f) return false;
+    V3 n, wo, alb;
+    float met;
+    shading_frame(S, p.cur, n, wo, alb, met);
+    float sPdf;
+    const BG bc = eval_ls<false>(wo, n, alb, met, clamp_rough(p.cur.rough), wi, sPdf, S.enable_metallic);
+    const V3 o = p.cur.pos + wi * kSelfOccOffset;
+    const V3 contrib = p.thr * bc.value * Li / fmaxf(envPdf, 1e-6f);
+    job.a = make_float4(o.x, o.y, o.z, wi.x);
+    job.b = make_float4(wi.y, wi.z, __uint_as_float(uint32_t(rng.state)), __uint_as_float(uint32_t(rng.state >> 32)));
+    job.c = make_float4(contrib.x, contrib.y, contrib.z, envPdf);
+    job.d = make_float4(sPdf, __uint_as_float(slot), 0.f, 0.f);
+    return true;
+}
+
+// The rest of sample_nee once the shadow ray is tested: the emission record of an unoccluded
+// ray, and the random stream after the test.
+__device__ __forceinline__ void nee_apply(Path& p, Rng& rng, const eng::ShadowJob& job) {
+    emit_clear(p);
+    rng.state = uint64_t(__float_as_uint(job.b.z)) | (uint64_t(__float_as_uint(job.b.w)) << 32);
+    if (job.d.z != 0.f) return;  // occluded
+    p.eray = Ray{v3(job.a.x, job.a.y, job.a.z), v3(job.a.w, job.b.x, job.b.y)};
+    p.elightPdf = job.c.w;
+    p.escatterPdf = job.d.x;
+    p.econtrib = v3(job.c.x, job.c.y, job.c.z);
+}
+
+// The NEE record of the tape after sample_nee; returns whether the path goes on to stage 1 (a
+// path that ends here has its outputs written).
+__device__ __forceinline__ bool nee_done(const DeviceScene& S, const FwdOut& out, uint32_t pix, const Path& p,
+                                         const Rng& rng, const Carry& c) {
     if (out.tape.base) {
         uint32_t* prec = out.tape.path(pix);
         prec[1] = uint32_t(rng.state);

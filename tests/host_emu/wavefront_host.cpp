@@ -1,7 +1,7 @@
 // Test harness: the two forward schedules of the tracer's device code, run on the CPU one
 // lane at a time over the launch of host_scene.hpp -- the one-lane-per-path body (fwd_path of
-// trace_path.hpp) and the stages of the wavefront schedule (wavefront_path.hpp) applied to each
-// path in turn -- and compared: every output, the visibility flags, the radiance-cache records
+// trace_path.hpp) and the stages of the wavefront schedule (wavefront_path.hpp, with the shadow
+// rays on the traversal engine of engine.hpp) applied to each path in turn -- and compared: every output, the visibility flags, the radiance-cache records
 // and the path tape (header, per-path records, every segment's hit list) must be equal bit for
 // bit.  Extra meta.txt keys: tape (0/1), tape_cap (hit entries per path), wave_traversal.
 // Prints "mismatches <n>" and exits with 1 when n > 0.
@@ -70,8 +70,8 @@ int main(int argc, char** argv) {
     // the wavefront stages, path by path
     // (each stage saves the path's state to a slot and the next loads it back: here slot = path)
     std::vector<float> state(size_t(wf::kWords) * P, 0.f);
-    wf::IO io{state.data(), state.data(), nullptr, nullptr, nullptr, P};
-    size_t staged = 0;
+    wf::IO io{state.data(), state.data(), nullptr, nullptr, nullptr, nullptr, nullptr, P};
+    size_t staged = 0, jobs = 0;
     la.for_each_path([&](const PixelId& px, int t) {
         const Lane L{lds.data() + t};
         const uint32_t pix = uint32_t(px.pix);
@@ -90,9 +90,18 @@ int main(int argc, char** argv) {
             c = cc;
         };
         if (go && la.S.enable_mis) {
+            // k_wf_nee_setup, k_wf_occlusion (the engine on this path's job alone), k_wf_nee_finish
             ++staged;
             hop(0);
-            go = wf::nee_path<16, kStackDeep>(la.S, Bo.fo, L, pix, p, rng, c);
+            eng::ShadowJob job;
+            if (wf::nee_setup(la.S, p, rng, pix, job)) {
+                ++jobs;
+                uint32_t head = 0u;
+                eng::occlusion<16, kStackDeep>(la.S, &job, 1u, &head, L);
+                wf::load(wf::slot_of(io, 0, pix), p, rng, c);  // the state as the finish kernel reads it
+                wf::nee_apply(p, rng, job);
+            }
+            go = wf::nee_done(la.S, Bo.fo, pix, p, rng, c);
         }
         for (uint32_t b = 1; go && b < la.S.max_bounces; ++b) {
             ++staged;
@@ -136,7 +145,7 @@ int main(int argc, char** argv) {
             }
         }
     }
-    printf("paths %zu, stage runs %zu, pool chunks %zu, stack overflow %d, mismatches %zu\n", P, staged, chunks,
-           la.error_flag, bad);
+    printf("paths %zu, stage runs %zu, shadow jobs %zu, pool chunks %zu, stack overflow %d, mismatches %zu\n", P, staged,
+           jobs, chunks, la.error_flag, bad);
     return bad || la.error_flag ? 1 : 0;
 }
