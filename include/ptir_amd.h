@@ -63,6 +63,11 @@ typedef struct ptir_options {
                                      union surface area; fewer nodes visited per ray, a longer build with
                                      a host synchronisation every 8 clustering rounds).  The trace results
                                      do not depend on it; a refit keeps the tree either builder made */
+    int32_t wide_walks;           /* which walks use the 4-wide nodes collapsed from the binary tree
+                                     (no counterpart in the reference): bit 0 the wave-coherent primary
+                                     walk, bit 1 the per-lane walks; the other walks use the binary
+                                     nodes.  Half the dependent node fetches, four boxes per fetch.
+                                     The trace results do not depend on it */
 } ptir_options;
 
 /* Scene inputs of one trace call (OptixTracer::trace argument list,
@@ -226,6 +231,12 @@ ptir_status ptir_build_bvh(ptir_tracer* tracer, const float* pos, const float* r
  * A counting launch (counters != NULL) reports and clears it too; a rebuild clears it.
  * The launch that overflowed dropped the subtrees it could not push. */
 ptir_status ptir_check(ptir_tracer* tracer, void* stream);
+
+/* Shape of the last built hierarchy (no counterpart in the reference, whose hierarchy is
+ * OptiX's own): the number of 4-wide nodes the traversal walks and the traversal stack
+ * entries its walks can need (at most 64; up to 36 runs the kernels' short LDS stack).  Either
+ * output may be NULL.  Synchronises nothing: the values were read back by the last rebuild. */
+ptir_status ptir_hierarchy_info(ptir_tracer* tracer, int32_t* wide_nodes, int32_t* stack_need);
 
 /* OptixTracer::trace (optixTracer.cpp:909-1013).  counters may be NULL; when not NULL
  * the kernels count work units into device memory and the call copies them back

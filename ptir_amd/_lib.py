@@ -22,7 +22,7 @@ PTIR_STATUS = {
     6: "PTIR_ERR_OUT_OF_MEMORY",
 }
 
-EXPORTED = ["ptir_create", "ptir_destroy", "ptir_build_bvh", "ptir_check", "ptir_grt_trace", "ptir_grt_trace_bwd",
+EXPORTED = ["ptir_create", "ptir_destroy", "ptir_build_bvh", "ptir_check", "ptir_hierarchy_info", "ptir_grt_trace", "ptir_grt_trace_bwd",
             "ptir_trace", "ptir_trace_bwd", "ptir_tape_bytes",
             "ptir_last_error", "ptir_version", "ptir_cache_create", "ptir_cache_destroy", "ptir_cache_num_weights",
             "ptir_cache_table_floats", "ptir_cache_read_params", "ptir_cache_write_params", "ptir_cache_read_grads",
@@ -37,7 +37,7 @@ class Options(C.Structure):
                 ("sph_max_degree", C.c_int32), ("enable_normals", C.c_int32), ("enable_hitcounts", C.c_int32),
                 ("enable_mis", C.c_int32), ("enable_metallic", C.c_int32), ("visualize_lights", C.c_int32),
                 ("kbuffer", C.c_int32), ("wave_traversal", C.c_int32), ("schedule", C.c_int32),
-                ("builder", C.c_int32)]
+                ("builder", C.c_int32), ("wide_walks", C.c_int32)]
 
 
 class Scene(C.Structure):
@@ -148,6 +148,7 @@ def lib() -> C.CDLL:
         L.ptir_trace_bwd_exact.argtypes = [C.c_void_p, C.POINTER(Scene), C.POINTER(Launch), C.POINTER(BwdInputs),
                                            C.POINTER(ExactGrads), C.POINTER(BwdOutputs), C.c_void_p]
         L.ptir_check.argtypes = [C.c_void_p, C.c_void_p]
+        L.ptir_hierarchy_info.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.ptir_grt_trace.argtypes = [C.c_void_p, C.POINTER(GrtScene), C.POINTER(Launch), C.POINTER(GrtOutputs),
                                      C.c_void_p]
         L.ptir_grt_trace_bwd.argtypes = [C.c_void_p, C.POINTER(GrtScene), C.POINTER(Launch), C.POINTER(GrtBwdInputs),
@@ -181,7 +182,7 @@ def lib() -> C.CDLL:
         L.ptir_resolve_backward.argtypes = [C.POINTER(Resolve)] + [vp] * 10
         L.ptir_adam_step.argtypes = [C.POINTER(AdamGroup), i32, C.c_float, C.c_float, C.c_float, i32, vp, vp]
         L.ptir_ragged_layout.argtypes = [vp, i32, vp, vp, vp]
-        for name in ("ptir_create", "ptir_destroy", "ptir_build_bvh", "ptir_check", "ptir_grt_trace",
+        for name in ("ptir_create", "ptir_destroy", "ptir_build_bvh", "ptir_check", "ptir_hierarchy_info", "ptir_grt_trace",
                      "ptir_grt_trace_bwd", "ptir_trace", "ptir_trace_bwd", "ptir_trace_bwd_exact",
                      "ptir_cache_create", "ptir_cache_destroy", "ptir_cache_read_params", "ptir_cache_write_params",
                      "ptir_cache_read_grads", "ptir_cache_query", "ptir_cache_backprop", "ptir_cache_step", "ptir_cache_train",

@@ -13,7 +13,9 @@ candidate configuration of the tracer and returns the fastest:
                  queues of live paths;
 * ``builder``    {"lbvh", "ploc"}: the hierarchy over the same Morton-ordered leaves, by radix
                  splits of the codes or by bottom-up clustering of smallest union area (fewer
-                 nodes per ray, a longer build; the timed step includes the build).
+                 nodes per ray, a longer build; the timed step includes the build);
+* ``wide_walks`` {"none", "all"}: every walk over the binary nodes, or over the 4-wide nodes
+                 collapsed from them (half the dependent node fetches, four boxes per fetch).
 
 All candidates compute the same paths (the same hits in the same order, the same decisions), so
 the choice changes speed only.  The block size is not searched: a block is two 8x8 wave tiles
@@ -39,6 +41,7 @@ KBUFFERS = (8, 16, 32)
 TRAVERSALS = ("wave", "lane")
 SCHEDULES = ("megakernel", "wavefront")
 BUILDERS = ("lbvh", "ploc")
+WIDE = ("none", "all")
 
 
 @dataclass
@@ -47,16 +50,19 @@ class TuneResult:
     traversal: str
     schedule: str
     builder: str
+    wide_walks: str
     ms: float
-    timings: dict = field(default_factory=dict)  # (kbuffer, traversal, schedule, builder) -> ms
+    timings: dict = field(default_factory=dict)  # (kbuffer, traversal, schedule, builder, wide_walks) -> ms
 
     def tracer_kwargs(self) -> dict:
-        return dict(kbuffer=self.kbuffer, traversal=self.traversal, schedule=self.schedule, builder=self.builder)
+        return dict(kbuffer=self.kbuffer, traversal=self.traversal, schedule=self.schedule, builder=self.builder,
+                    wide_walks=self.wide_walks)
 
     def as_dict(self) -> dict:
         return {"kbuffer": self.kbuffer, "traversal": self.traversal, "schedule": self.schedule,
-                "builder": self.builder, "ms": round(self.ms, 3),
-                "timings_ms": {f"k{k}/{t}/{s}/{b}": round(v, 3) for (k, t, s, b), v in sorted(self.timings.items())}}
+                "builder": self.builder, "wide_walks": self.wide_walks, "ms": round(self.ms, 3),
+                "timings_ms": {f"k{k}/{t}/{s}/{b}/{w}": round(v, 3)
+                               for (k, t, s, b, w), v in sorted(self.timings.items())}}
 
 
 def _time(fn, reps: int) -> float:
@@ -74,8 +80,9 @@ def _time(fn, reps: int) -> float:
 
 def autotune(geometry, scene_args, frame, ray_to_world, ray_ori, ray_dir, *, tracer_kwargs=None, trace_kwargs=None,
              backward: bool = True, reps: int = 3, kbuffers=KBUFFERS, traversals=TRAVERSALS,
-             schedules=SCHEDULES, builders=BUILDERS) -> TuneResult:
-    """Time every (kbuffer, traversal, schedule, builder) on one workload and return the fastest.
+             schedules=SCHEDULES, builders=BUILDERS, wide=WIDE) -> TuneResult:
+    """Time every (kbuffer, traversal, schedule, builder, wide_walks) on one workload and return the
+    fastest.
 
     geometry: (positions, rotations, scales, densities) for build_bvh; scene_args: the six scene
     tensors of HipTracer.trace (particle density, material, radiance, shading normal,
@@ -86,8 +93,8 @@ def autotune(geometry, scene_args, frame, ray_to_world, ray_ori, ray_dir, *, tra
     trace_kwargs = dict(trace_kwargs or {})
     max_bounces = int(trace_kwargs.get("max_bounces", 3))
     timings = {}
-    for k, trav, sched, bld in itertools.product(kbuffers, traversals, schedules, builders):
-        tr = HipTracer(kbuffer=k, traversal=trav, schedule=sched, builder=bld, **tracer_kwargs)
+    for k, trav, sched, bld, ww in itertools.product(kbuffers, traversals, schedules, builders, wide):
+        tr = HipTracer(kbuffer=k, traversal=trav, schedule=sched, builder=bld, wide_walks=ww, **tracer_kwargs)
         tape = tr.new_tape(ray_dir, max_bounces)
 
         def step():
@@ -100,7 +107,7 @@ def autotune(geometry, scene_args, frame, ray_to_world, ray_ori, ray_dir, *, tra
                              None, None, None, None, None, None, None, None, None, None, None, None, None, None,
                              grad, None, tape=tape, **bwd_kw)
 
-        timings[(k, trav, sched, bld)] = _time(step, reps)
+        timings[(k, trav, sched, bld, ww)] = _time(step, reps)
         del tr, tape
     best = min(timings, key=timings.get)
-    return TuneResult(best[0], best[1], best[2], best[3], timings[best], timings)
+    return TuneResult(best[0], best[1], best[2], best[3], best[4], timings[best], timings)

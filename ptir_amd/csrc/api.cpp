@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -16,6 +17,8 @@ struct ptir_tracer {
     ptir::BvhBuffers bvh;
     bool built = false;
     bool short_stack = false;  // the built hierarchy fits the short traversal stack
+    int32_t stack_need = 0;    // traversal stack entries the built hierarchy needs
+    int32_t wide_nodes = 0;    // 4-wide nodes of the built hierarchy
     void* wf = nullptr;        // wavefront-schedule workspace
     uint64_t wf_bytes = 0;
     std::string err;
@@ -92,6 +95,7 @@ ptir::DeviceScene make_scene(ptir_tracer* t, const ptir_scene* sc, const ptir_la
     S.srec = b.srec;
     S.prec = b.prec;
     S.nodes = b.nodes;
+    S.wnodes = b.wide;
     S.bounds = b.bounds;
     S.sh = sc->particle_radiance;
     S.env = reinterpret_cast<const float4*>(sc->environment);
@@ -136,6 +140,7 @@ ptir::DeviceScene make_scene(ptir_tracer* t, const ptir_scene* sc, const ptir_la
     S.enable_mis = t->opts.enable_mis;
     S.enable_metallic = t->opts.enable_metallic;
     S.has_snormal = sc->shading_normal != nullptr;
+    S.wide_walks = t->opts.wide_walks;
     S.wave_traversal = t->opts.wave_traversal;
     S.counters = nullptr;
     S.error_flag = b.error_flag;
@@ -208,6 +213,8 @@ ptir_status ptir_create(const ptir_options* options, ptir_tracer** out) {
         return fail(nullptr, PTIR_ERR_INVALID_ARGUMENT, "kbuffer must be 8, 16 or 32");
     if (options->wave_traversal != 0 && options->wave_traversal != 1)
         return fail(nullptr, PTIR_ERR_INVALID_ARGUMENT, "wave_traversal must be 0 or 1");
+    if (options->wide_walks < 0 || options->wide_walks > 3)
+        return fail(nullptr, PTIR_ERR_INVALID_ARGUMENT, "wide_walks must be in [0,3]");
     if (options->schedule != 0 && options->schedule != 1)
         return fail(nullptr, PTIR_ERR_INVALID_ARGUMENT, "schedule must be 0 (one lane per path) or 1 (wavefront)");
     if (options->builder != 0 && options->builder != 1)
@@ -245,13 +252,18 @@ ptir_status ptir_build_bvh(ptir_tracer* t, const float* pos, const float* rot, c
     hipError_t e = ptir::bvh_build(t->bvh, kopts(t->opts), pos, rot, scl, dns, n, rebuild != 0, t->opts.builder, s);
     if (e != hipSuccess) return hip_fail(t, e, "bvh_build");
     if (rebuild) {
-        int32_t depth = 0;
+        int32_t need = 0, wide_count = 0, depth = 0;
         e = ptir::fill_zero(t->bvh.error_flag, sizeof(int32_t), s);
-        if (e == hipSuccess) e = ptir::bvh_depth(t->bvh, s, &depth);
-        if (e != hipSuccess) return hip_fail(t, e, "bvh_depth");
-        if (depth + 1 > ptir::kStackDeep)
-            return fail(t, PTIR_ERR_BVH_DEPTH, "hierarchy depth " + std::to_string(depth) + " exceeds the traversal stack");
-        t->short_stack = depth + 1 <= ptir::kStackShort;
+        if (e == hipSuccess) e = ptir::bvh_stack_need(t->bvh, s, &need, &wide_count, &depth);
+        if (e != hipSuccess) return hip_fail(t, e, "bvh_stack_need");
+        // a binary walk pushes at most one entry per level
+        need = std::max(need, depth + 1);
+        if (need > ptir::kStackDeep)
+            return fail(t, PTIR_ERR_BVH_DEPTH, "the hierarchy's walks need " + std::to_string(need) +
+                                                   " stack entries, more than the traversal stack holds");
+        t->short_stack = need <= ptir::kStackShort;
+        t->stack_need = need;
+        t->wide_nodes = n > 1 ? wide_count : 0;
     }
     t->built = true;
     return PTIR_OK;
@@ -269,6 +281,14 @@ ptir_status ptir_check(ptir_tracer* t, void* stream) {
     e = ptir::fill_zero(t->bvh.error_flag, sizeof(int32_t), s);
     if (e != hipSuccess) return hip_fail(t, e, "clearing the error flag");
     return fail(t, PTIR_ERR_BVH_DEPTH, "a traversal stack overflowed in a launch since the last check");
+}
+
+ptir_status ptir_hierarchy_info(ptir_tracer* t, int32_t* wide_nodes, int32_t* stack_need) {
+    if (!t) return fail(nullptr, PTIR_ERR_INVALID_ARGUMENT, "null tracer");
+    if (!t->built) return fail(t, PTIR_ERR_NOT_BUILT, "no hierarchy built");
+    if (wide_nodes) *wide_nodes = t->wide_nodes;
+    if (stack_need) *stack_need = t->stack_need;
+    return PTIR_OK;
 }
 
 ptir_status ptir_trace(ptir_tracer* t, const ptir_scene* sc, const ptir_launch* la, const ptir_fwd_outputs* out,

@@ -7,9 +7,13 @@
 //      preserving integer atomics);
 //   2. k_morton + radix sort: 63-bit Morton codes of the centres;
 //   3. k_karras: binary radix tree (Karras, HPG 2012), duplicate codes split by index;
-//   4. k_refit: bottom-up box fit with one arrival counter per node;
-//   5. k_depth: hierarchy depth, checked against the traversal stack.
-// Node layout (64 B): {lo_l.xyz, link_l} {hi_l.xyz, link_r} {lo_r.xyz, -} {hi_r.xyz, -};
+//   4. k_refit: bottom-up box fit with one arrival counter per node (or PLOC, ploc.hpp,
+//      in place of 3 and 4);
+//   5. the 4-wide hierarchy the traversal walks (wide.hpp): even-depth nodes flagged and
+//      numbered by a prefix sum, filled from their grandchildren; a refit only refills;
+//   6. k_wide_need, k_depth: traversal stack entries the 4-wide and the binary walks need,
+//      checked against the traversal stack.
+// Binary node layout (64 B): {lo_l.xyz, link_l} {hi_l.xyz, link_r} {lo_r.xyz, -} {hi_r.xyz, -};
 // link >= 0 is an internal node, link < 0 is leaf slot ~link.
 #include <hipcub/hipcub.hpp>
 
@@ -197,15 +201,37 @@ __global__ void __launch_bounds__(256) k_refit(const float4* __restrict__ aabb, 
     }
 }
 
-// Depth of the deepest leaf; the walk stops at kDepthCap, which the build then rejects
-// (a malformed parent chain cannot loop).
-constexpr int kDepthCap = 4096;
+// ---------------------------------------------------------------- 4-wide hierarchy (wide.hpp)
+__global__ void __launch_bounds__(256) k_wide_flag(const int32_t* __restrict__ parent, int m, uint32_t* __restrict__ flag) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) flag[i] = wide::is_wide(parent, i);
+}
+__global__ void __launch_bounds__(256) k_wide_number(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pre,
+                                                     int m, int32_t* __restrict__ wmap) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) wide::number(flag, pre, i, wmap);
+}
+__global__ void __launch_bounds__(256) k_wide_fill(const float4* __restrict__ bin, const int32_t* __restrict__ wmap, int m,
+                                                   float4* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) wide::fill(bin, wmap, i, out);
+}
+// Depth of the deepest leaf of the binary tree; the walk stops at wide::kDepthCap, which the
+// stack check then rejects (a malformed parent chain cannot loop).
 __global__ void __launch_bounds__(256) k_depth(const int32_t* __restrict__ parent, int n, int32_t* depth) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     int d = 0;
-    for (int p = parent[(n - 1) + j]; p >= 0 && d < kDepthCap; p = parent[p]) ++d;
+    for (int p = parent[(n - 1) + j]; p >= 0 && d < wide::kDepthCap; p = parent[p]) ++d;
     atomicMax(depth, d);
+}
+__global__ void k_wide_count(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pre, int m, int32_t* count) {
+    *count = int32_t(pre[m - 1] + flag[m - 1]);
+}
+__global__ void __launch_bounds__(256) k_wide_need(const int32_t* __restrict__ parent, const int32_t* __restrict__ wmap,
+                                                   const float4* __restrict__ out, int m, int32_t* need) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) atomicMax(need, wide::stack_need(parent, wmap, out, i));
 }
 
 // ---------------------------------------------------------------- PLOC builder (ploc.hpp)
@@ -339,6 +365,10 @@ hipError_t bvh_reserve(BvhBuffers& b, uint32_t n) {
     G(b.idx, N);
     G(b.leaf_id, N);
     G(b.nodes, 4 * M);
+    G(b.wide, size_t(wide::kWords) * M);
+    G(b.wflag, M);
+    G(b.wpre, M);
+    G(b.wmap, M);
     G(b.parent, 2 * N);
     G(b.flags, M);
     G(b.grec, 4 * N);
@@ -349,7 +379,7 @@ hipError_t bvh_reserve(BvhBuffers& b, uint32_t n) {
     if (!b.bounds_i) {
         if ((e = hipMalloc((void**)&b.bounds_i, 6 * sizeof(int32_t))) != hipSuccess) return e;
         if ((e = hipMalloc((void**)&b.bounds, 6 * sizeof(float))) != hipSuccess) return e;
-        if ((e = hipMalloc((void**)&b.depth, sizeof(int32_t))) != hipSuccess) return e;
+        if ((e = hipMalloc((void**)&b.wneed, 3 * sizeof(int32_t))) != hipSuccess) return e;
         if ((e = hipMalloc((void**)&b.counters, size_t(kCounterCopies) * kCounterStride * sizeof(unsigned long long))) !=
             hipSuccess)
             return e;
@@ -365,15 +395,24 @@ hipError_t bvh_reserve(BvhBuffers& b, uint32_t n) {
         if ((e = hipMalloc(&b.sort_tmp, tmp)) != hipSuccess) return e;
         b.sort_tmp_bytes = tmp;
     }
+    tmp = 0;
+    e = hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, b.wflag, b.wpre, int(M));
+    if (e != hipSuccess) return e;
+    if (tmp > b.wscan_tmp_bytes) {
+        if (b.wscan_tmp) (void)hipFree(b.wscan_tmp);
+        b.wscan_tmp = nullptr;
+        if ((e = hipMalloc(&b.wscan_tmp, tmp + 256)) != hipSuccess) return e;
+        b.wscan_tmp_bytes = tmp + 256;
+    }
     b.cap = n;
     return hipSuccess;
 }
 
 void bvh_free(BvhBuffers& b) {
     void* ptrs[] = {b.aabb, b.keys, b.keys_sorted, b.idx, b.leaf_id, b.nodes, b.parent, b.flags, b.bounds_i,
-                    b.bounds, b.depth, b.sort_tmp, b.grec, b.srec, b.prec, b.dmat_leaf, b.counters, b.error_flag,
+                    b.bounds, b.sort_tmp, b.grec, b.srec, b.prec, b.dmat_leaf, b.counters, b.error_flag,
                     b.tape_state, b.pc[0], b.pc[1], b.pb[0], b.pb[1], b.pnn, b.pval, b.pscan, b.pstate, b.pbad,
-                    b.scan_tmp};
+                    b.scan_tmp, b.wide, b.wflag, b.wpre, b.wmap, b.wneed, b.wscan_tmp};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     b = BvhBuffers();
@@ -462,26 +501,46 @@ hipError_t bvh_build(BvhBuffers& b, const KernelOpts& o, const float* pos, const
         e = hipcub::DeviceRadixSort::SortPairs(b.sort_tmp, b.sort_tmp_bytes, b.keys, b.keys_sorted, b.idx, b.leaf_id,
                                                int(n), 0, 63, s);
         if (e != hipSuccess) return e;
-        if (n > 1 && builder == 1) {
-            bool done = false;
-            if ((e = ploc_build(b, n, s, &done)) != hipSuccess) return e;
-            if (done) return hipGetLastError();  // nodes, boxes and parents are written
+        bool done = false;
+        if (n > 1 && builder == 1 && (e = ploc_build(b, n, s, &done)) != hipSuccess) return e;
+        // PLOC writes nodes, boxes and parents; otherwise the radix tree, then its boxes
+        if (n > 1 && !done) {
+            k_karras<<<blocks(n - 1, 256), 256, 0, s>>>(b.keys_sorted, int(n), b.nodes, b.parent, b.flags);
+            k_refit<<<blocks(n, 256), 256, 0, s>>>(b.aabb, b.leaf_id, int(n), b.nodes, b.parent, b.flags);
         }
-        if (n > 1) k_karras<<<blocks(n - 1, 256), 256, 0, s>>>(b.keys_sorted, int(n), b.nodes, b.parent, b.flags);
     } else if (n > 1) {
         k_clear_flags<<<blocks(n - 1, 256), 256, 0, s>>>(b.flags, int(n - 1));
+        k_refit<<<blocks(n, 256), 256, 0, s>>>(b.aabb, b.leaf_id, int(n), b.nodes, b.parent, b.flags);
     }
-    if (n > 1) k_refit<<<blocks(n, 256), 256, 0, s>>>(b.aabb, b.leaf_id, int(n), b.nodes, b.parent, b.flags);
+    if (n > 1) {
+        const int m = int(n - 1);
+        if (rebuild) {
+            k_wide_flag<<<blocks(m, 256), 256, 0, s>>>(b.parent, m, b.wflag);
+            size_t tmp = b.wscan_tmp_bytes;
+            if ((e = hipcub::DeviceScan::ExclusiveSum(b.wscan_tmp, tmp, b.wflag, b.wpre, m, s)) != hipSuccess) return e;
+            k_wide_number<<<blocks(m, 256), 256, 0, s>>>(b.wflag, b.wpre, m, b.wmap);
+        }
+        k_wide_fill<<<blocks(m, 256), 256, 0, s>>>(b.nodes, b.wmap, m, b.wide);
+    }
     return hipGetLastError();
 }
 
-hipError_t bvh_depth(BvhBuffers& b, hipStream_t s, int32_t* host_depth) {
-    hipError_t e = fill_zero(b.depth, sizeof(int32_t), s);
+hipError_t bvh_stack_need(BvhBuffers& b, hipStream_t s, int32_t* host_need, int32_t* host_count, int32_t* host_depth) {
+    hipError_t e = fill_zero(b.wneed, 3 * sizeof(int32_t), s);
     if (e != hipSuccess) return e;
-    if (b.n > 1) k_depth<<<blocks(b.n, 256), 256, 0, s>>>(b.parent, int(b.n), b.depth);
-    e = hipMemcpyAsync(host_depth, b.depth, sizeof(int32_t), hipMemcpyDeviceToHost, s);
+    if (b.n > 1) {
+        k_wide_need<<<blocks(b.n - 1, 256), 256, 0, s>>>(b.parent, b.wmap, b.wide, int(b.n - 1), b.wneed);
+        k_wide_count<<<1, 1, 0, s>>>(b.wflag, b.wpre, int(b.n - 1), b.wneed + 1);
+        k_depth<<<blocks(b.n, 256), 256, 0, s>>>(b.parent, int(b.n), b.wneed + 2);
+    }
+    int32_t h[3] = {0, 0, 0};
+    e = hipMemcpyAsync(h, b.wneed, sizeof(h), hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return e;
-    return hipStreamSynchronize(s);
+    e = hipStreamSynchronize(s);
+    *host_need = h[0];
+    *host_count = h[1];
+    *host_depth = h[2];
+    return e;
 }
 
 hipError_t prep_records(BvhBuffers& b, const KernelOpts& o, const float* density, const float* material,

@@ -98,7 +98,8 @@ struct DeviceScene {
     const float4* grec;       // [N*4]   GaussRec, leaf order
     const float4* srec;       // [N*2]   ShadeRec, leaf order
     const float4* prec;       // [N]     PrimRec, leaf order
-    const float4* nodes;      // [(N-1)*4] internal nodes: child boxes + child links
+    const float4* nodes;      // [(N-1)*4] binary internal nodes: child boxes + child links
+    const float4* wnodes;     // 4-wide nodes collapsed from them (wide.hpp)
     const float* bounds;      // [6] scene AABB lo.xyz hi.xyz (device)
     const float* sh;          // [N, sh_stride] original order
     const float4* env;        // [env_h, env_w]
@@ -118,6 +119,7 @@ struct DeviceScene {
     int32_t B, H, W;
     int32_t enable_normals, enable_hitcounts, enable_mis, enable_metallic, has_snormal;
     int32_t wave_traversal;  // primary rays walk the hierarchy per wave (wave_collect)
+    int32_t wide_walks;      // bit 0: wave walks use the 4-wide nodes; bit 1: per-lane walks do
     int32_t tiles_x, tiles_y, num_blocks;  // launch tiling
     // ragged batch: num_images > 0 images of their own size, table rows (H, W, first tile,
     // first path); B, H, W are then (1, 1, total paths)

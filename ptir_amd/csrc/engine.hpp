@@ -72,9 +72,41 @@ struct Walk {
     // One node phase and one leaf phase; every active lane of the wave calls together (a lane
     // with no walk stands by).
     __device__ __forceinline__ void burst(const DeviceScene& S, const Ray& r, const Lane& L) {
+        const bool wide_lane = (S.wide_walks & 2) != 0;
         for (;;) {
             if (__ballot(walk && q0 < 0) == 0) break;  // every walking lane has a leaf queued
-            if (walk && q2 < 0) {                         // room for both children
+            if (wide_lane && walk && q0 < 0) {
+                const float4* nd = S.wnodes + wide::kWords * link;
+                count(S, C_NODES);
+                float tn[4];
+                int32_t cl[4];
+                const uint32_t m = wide::test4(nd, r.o, inv, tmin, fminf(tmax, bt[K - 1]), tn);
+                wide::links4(nd, cl);
+                int ni = 0;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const bool hit = ((m >> c) & 1u) != 0u && cl[c] != wide::kEmpty;
+                    if (hit && cl[c] < 0) push(~cl[c]);
+                    const bool in = hit && cl[c] >= 0;
+                    ni += in ? 1 : 0;
+                    tn[c] = in ? tn[c] : __int_as_float(0x7f800000);
+                }
+                if (ni == 0) {
+                    if (sp == 0) walk = false;
+                    else link = int(L.at(--sp));
+                } else {
+                    wide::sort4(tn, cl);
+                    if (sp + ni - 1 > SD) {  // cannot happen after the build-time stack check
+                        atomicOr(S.error_flag, 1);
+                        walk = false;
+                    } else {
+#pragma unroll
+                        for (int j = 3; j >= 1; --j)
+                            if (j < ni) L.at(sp++) = uint32_t(cl[j]);
+                        link = cl[0];
+                    }
+                }
+            } else if (!wide_lane && walk && q2 < 0) {    // binary nodes: room for both children
                 const float4* nd = S.nodes + 4 * link;
                 const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2], n3 = nd[3];
                 count(S, C_NODES);

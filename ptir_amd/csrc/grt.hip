@@ -6,8 +6,10 @@ namespace ptir {
 
 template <int K, int SD>
 __global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(waves_per_eu(K, SD))))
-k_grt_fwd(DeviceScene S, grt::FwdOut out, const float4* __restrict__ nodes, const float4* __restrict__ grec) {
+k_grt_fwd(DeviceScene S, grt::FwdOut out, const float4* __restrict__ nodes, const float4* __restrict__ wnodes,
+          const float4* __restrict__ grec) {
     S.nodes = nodes;
+    S.wnodes = wnodes;
     S.grec = grec;
     __shared__ uint32_t lds[LdsLayout<K, SD>::kWords * kTraceBlock];
     __shared__ uint32_t wst[(kTraceBlock / 64) * kStackDeep];
@@ -20,8 +22,10 @@ k_grt_fwd(DeviceScene S, grt::FwdOut out, const float4* __restrict__ nodes, cons
 
 template <int K, int SD>
 __global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(waves_per_eu(K, SD))))
-k_grt_bwd(DeviceScene S, grt::BwdIO io, const float4* __restrict__ nodes, const float4* __restrict__ grec) {
+k_grt_bwd(DeviceScene S, grt::BwdIO io, const float4* __restrict__ nodes, const float4* __restrict__ wnodes,
+          const float4* __restrict__ grec) {
     S.nodes = nodes;
+    S.wnodes = wnodes;
     S.grec = grec;
     __shared__ uint32_t lds[LdsLayout<K, SD>::kWords * kTraceBlock];
     __shared__ uint32_t wst[(kTraceBlock / 64) * kStackDeep];
@@ -39,7 +43,7 @@ struct GrtFwdLaunch {
     hipStream_t s;
     template <int K, int SD>
     void run() const {
-        k_grt_fwd<K, SD><<<S.num_blocks, kTraceBlock, 0, s>>>(S, out, S.nodes, S.grec);
+        k_grt_fwd<K, SD><<<S.num_blocks, kTraceBlock, 0, s>>>(S, out, S.nodes, S.wnodes, S.grec);
     }
 };
 struct GrtBwdLaunch {
@@ -48,7 +52,7 @@ struct GrtBwdLaunch {
     hipStream_t s;
     template <int K, int SD>
     void run() const {
-        k_grt_bwd<K, SD><<<S.num_blocks, kTraceBlock, 0, s>>>(S, io, S.nodes, S.grec);
+        k_grt_bwd<K, SD><<<S.num_blocks, kTraceBlock, 0, s>>>(S, io, S.nodes, S.wnodes, S.grec);
     }
 };
 }  // namespace

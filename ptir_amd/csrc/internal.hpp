@@ -9,6 +9,7 @@
 
 #include "device.hpp"
 #include "ploc.hpp"
+#include "wide.hpp"
 
 namespace ptir {
 
@@ -26,12 +27,18 @@ struct BvhBuffers {
     unsigned long long* keys_sorted = nullptr;
     uint32_t* idx = nullptr;         // [N]
     uint32_t* leaf_id = nullptr;     // [N] leaf slot -> original index
-    float4* nodes = nullptr;         // [(N-1)*4]
+    float4* nodes = nullptr;         // [(N-1)*4] binary nodes (LBVH or PLOC)
+    float4* wide = nullptr;          // [(N-1)*8] 4-wide nodes collapsed from them (the traversal's)
+    uint32_t* wflag = nullptr;       // [N-1] binary node is a 4-wide node
+    uint32_t* wpre = nullptr;        // [N-1] exclusive prefix sum of wflag
+    int32_t* wmap = nullptr;         // [N-1] binary node -> 4-wide node or -1
+    int32_t* wneed = nullptr;        // [3] stack entries the 4-wide walks need, 4-wide node count, binary depth
+    void* wscan_tmp = nullptr;
+    size_t wscan_tmp_bytes = 0;
     int32_t* parent = nullptr;       // [2N-1]
     uint32_t* flags = nullptr;       // [N-1]
     int32_t* bounds_i = nullptr;     // [6] order-preserving int encoding of the scene box
     float* bounds = nullptr;         // [6]
-    int32_t* depth = nullptr;        // [1]
     void* sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
     // per-trace records (leaf order)
@@ -59,7 +66,9 @@ hipError_t bvh_reserve(BvhBuffers& b, uint32_t n);
 void bvh_free(BvhBuffers& b);
 hipError_t bvh_build(BvhBuffers& b, const KernelOpts& o, const float* pos, const float* rot, const float* scl,
                      const float* dns, uint32_t n, bool rebuild, int builder, hipStream_t s);
-hipError_t bvh_depth(BvhBuffers& b, hipStream_t s, int32_t* host_depth);
+// Traversal stack entries the built 4-wide hierarchy needs, its node count and the depth of
+// the binary one (synchronises the stream).
+hipError_t bvh_stack_need(BvhBuffers& b, hipStream_t s, int32_t* host_need, int32_t* host_count, int32_t* host_depth);
 hipError_t prep_records(BvhBuffers& b, const KernelOpts& o, const float* density, const float* material,
                         const float* snormal, hipStream_t s);
 hipError_t scatter_material_grad(const BvhBuffers& b, float* dmat, hipStream_t s);
@@ -116,8 +125,9 @@ hipError_t launch_trace_adjoint(const DeviceScene& S, const AdjIO& io, int kbuf,
 
 constexpr int kTraceBlock = 128;  // 2 waves: a 16x8 pixel tile
 constexpr int kStackDeep = 64;    // traversal stack entries per lane (LDS): any accepted hierarchy
-// hierarchies of depth < 36 run with this one: 36 words per lane of LDS (with the wave stacks,
-// 18.9 KB per 128-lane block) still fit 8 blocks = four waves per SIMD in 160 KB of LDS
+// hierarchies whose walks need at most 36 stack entries run with this one: 36 words per lane
+// of LDS (with the wave stacks, 18.9 KB per 128-lane block) still fit 8 blocks = four waves per
+// SIMD in 160 KB of LDS
 constexpr int kStackShort = 36;
 
 // bytes of a tape with `segs` segments per path and `cap` hits per path

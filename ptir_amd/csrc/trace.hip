@@ -15,8 +15,10 @@ template <int K, int SD>
 __global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(waves_per_eu(K, SD))))
 k_trace_fwd(DeviceScene S, FwdOut out,
                                                           const float4* __restrict__ nodes,
+                                                          const float4* __restrict__ wnodes,
                                                           const float4* __restrict__ grec) {
     S.nodes = nodes;
+    S.wnodes = wnodes;
     S.grec = grec;
     __shared__ uint32_t lds[LdsLayout<K, SD>::kWords * kTraceBlock];
     __shared__ uint32_t wst[(kTraceBlock / 64) * kStackDeep];  // wave-uniform node stacks
@@ -118,7 +120,7 @@ struct FwdLaunch {
     hipStream_t s;
     template <int K, int SD>
     void run() const {
-        k_trace_fwd<K, SD><<<S.num_blocks, kTraceBlock, 0, s>>>(S, out, S.nodes, S.grec);
+        k_trace_fwd<K, SD><<<S.num_blocks, kTraceBlock, 0, s>>>(S, out, S.nodes, S.wnodes, S.grec);
     }
 };
 

@@ -9,8 +9,9 @@
 // that neighbouring tiles land on the same XCD (and its L2).
 //
 // Hit collection replaces OptiX's any-hit k-buffer (referenceOptix.cu:95-133): a
-// stack-based LBVH walk keeps the K closest candidates with t* in (tmin, tmax) sorted
-// by (t*, leaf slot) in registers and prunes boxes that start beyond the K-th distance.
+// stack-based walk of the 4-wide hierarchy (wide.hpp) keeps the K closest candidates with
+// t* in (tmin, tmax) sorted by (t*, leaf slot) in registers and prunes boxes that start
+// beyond the K-th distance.
 // The traversal stack lives in LDS; once the walk ends the same LDS words hold the
 // sorted K-buffer, which the compositing loop then consumes in order.
 #pragma once
@@ -100,13 +101,83 @@ __device__ __forceinline__ int collect(const DeviceScene& S, const Ray& r, float
     const V3 inv = v3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
     if (S.n > 0 && S.root < 0) {
         leaf(~S.root);
-    } else if (S.n > 0) {
+    } else if (S.n > 0 && (S.wide_walks & 2)) {
         // Node steps and leaf tests run in separate phases so that each phase keeps most
-        // lanes of the wave busy.  A lane queues the leaves it reaches (up to four) and
-        // keeps walking while it has room; the node phase ends once every lane still
-        // walking has a leaf queued, then all queues drain together.  The K nearest
-        // candidates do not depend on the order of the tests; culling against the
-        // K-th distance only gets later, never wrong.
+        // lanes of the wave busy.  A lane with an empty leaf queue steps through one 4-wide
+        // node: the leaves it enters join the queue (at most four), the inner children it
+        // enters are taken nearest first (the others go on the stack, farthest deepest).  The
+        // node phase ends once every lane still walking has a leaf queued, then all queues
+        // drain together.  The K nearest candidates do not depend on the order of the tests;
+        // culling against the K-th distance only gets later, never wrong.
+        int link = S.root;
+        int sp = 0;
+        bool walk = true;             // nodes left to visit
+        int q0 = -1, q1 = -1, q2 = -1, q3 = -1, q4 = -1, q5 = -1, q6 = -1, q7 = -1;  // queued leaf slots
+        auto push = [&](int s) {
+            q7 = q6;
+            q6 = q5;
+            q5 = q4;
+            q4 = q3;
+            q3 = q2;
+            q2 = q1;
+            q1 = q0;
+            q0 = s;
+        };
+        for (;;) {
+            for (;;) {
+                if (__ballot(walk && q0 < 0) == 0) break;  // every walking lane has a leaf queued
+                if (walk && q4 < 0) {  // room for four leaves
+                    const float4* nd = S.wnodes + wide::kWords * link;
+                    count(S, C_NODES);
+                    float tn[4];
+                    int32_t cl[4];
+                    const uint32_t m = wide::test4(nd, r.o, inv, tmin, fminf(tmax, bt[K - 1]), tn);
+                    wide::links4(nd, cl);
+                    int ni = 0;
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        const bool hit = ((m >> c) & 1u) != 0u && cl[c] != wide::kEmpty;
+                        if (hit && cl[c] < 0) push(~cl[c]);
+                        const bool in = hit && cl[c] >= 0;
+                        ni += in ? 1 : 0;
+                        tn[c] = in ? tn[c] : __int_as_float(0x7f800000);
+                    }
+                    if (ni == 0) {
+                        if (sp == 0) walk = false;
+                        else link = int(L.at(--sp));
+                    } else {
+                        wide::sort4(tn, cl);
+                        if (sp + ni - 1 > SD) {  // cannot happen after the build-time stack check
+                            atomicOr(S.error_flag, 1);
+                            walk = false;
+                        } else {
+#pragma unroll
+                            for (int j = 3; j >= 1; --j)
+                                if (j < ni) L.at(sp++) = uint32_t(cl[j]);
+                            link = cl[0];
+                        }
+                    }
+                }
+            }
+            while (__ballot(q0 >= 0) != 0) {
+                if (q0 >= 0) {
+                    const int slot = q0;
+                    q0 = q1;
+                    q1 = q2;
+                    q2 = q3;
+                    q3 = q4;
+                    q4 = q5;
+                    q5 = q6;
+                    q6 = q7;
+                    q7 = -1;
+                    leaf(slot);
+                }
+            }
+            if (__ballot(walk) == 0) break;
+        }
+    } else if (S.n > 0) {
+        // The same phases over the binary nodes: a lane queues the leaves it reaches (up to
+        // four) and keeps walking while it has room for both children.
         int link = S.root;
         int sp = 0;
         bool walk = true;             // nodes left to visit
@@ -705,6 +776,56 @@ __device__ __forceinline__ int wave_collect(const DeviceScene& S, const Ray& r, 
     if (S.n == 0) need = false;
     if (S.n > 0 && S.root < 0) {
         leaf(need, ~S.root);
+    } else if (__ballot(need) != 0 && (S.wide_walks & 1)) {
+        int link = S.root;  // wave-uniform
+        int sp = 0;         // wave-uniform
+        for (;;) {
+            link = __builtin_amdgcn_readfirstlane(link);
+            const float4* nd = S.wnodes + wide::kWords * link;
+            if (need) count(S, C_NODES);
+            float tn[4];
+            int32_t cl[4];
+            const uint32_t m = need ? wide::test4(nd, r.o, inv, tmin, fminf(tmax, bt[K - 1]), tn) : 0u;
+            wide::links4(nd, cl);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) cl[c] = __builtin_amdgcn_readfirstlane(cl[c]);
+            // leaves that some lane enters: every lane tests them (a lane counts a leaf only
+            // when its own ray enters the box)
+            uint32_t go = 0u;  // wave-uniform: inner children that some lane enters
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const bool hit = ((m >> c) & 1u) != 0u;
+                const bool any = __ballot(hit) != 0 && cl[c] != wide::kEmpty;
+                if (any && cl[c] < 0) leaf(hit, ~cl[c]);
+                if (any && cl[c] >= 0) go |= 1u << c;
+            }
+            if (go == 0u) {
+                if (sp == 0) break;
+                --sp;
+                link = int(wstack[sp]);
+                continue;
+            }
+            // order of the inner children: entry distance as seen by the first lane that enters
+            // one of them (children it does not enter come after, in child order)
+            const int leader = __ffsll((long long)__ballot((m & go) != 0u)) - 1;
+            float key[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const float mine = ((m >> c) & 1u) != 0u ? tn[c] : 3.0e38f;
+                const float lead = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine), leader));
+                key[c] = ((go >> c) & 1u) != 0u ? lead : __int_as_float(0x7f800000);
+            }
+            wide::sort4(key, cl);
+            const int ni = __popc(go);
+            if (sp + ni - 1 > kStackDeep) {  // cannot happen after the build-time stack check
+                atomicOr(S.error_flag, 1);
+                break;
+            }
+#pragma unroll
+            for (int j = 3; j >= 1; --j)
+                if (j < ni) wstack[sp++] = uint32_t(cl[j]);  // every lane stores the same uniform value
+            link = cl[0];
+        }
     } else if (__ballot(need) != 0) {
         int link = S.root;  // wave-uniform
         int sp = 0;         // wave-uniform

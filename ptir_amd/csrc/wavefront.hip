@@ -6,8 +6,10 @@ namespace ptir {
 
 template <int K, int SD>
 __global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(waves_per_eu(K, SD))))
-k_wf_primary(DeviceScene S, FwdOut out, wf::IO io, const float4* __restrict__ nodes, const float4* __restrict__ grec) {
+k_wf_primary(DeviceScene S, FwdOut out, wf::IO io, const float4* __restrict__ nodes, const float4* __restrict__ wnodes,
+          const float4* __restrict__ grec) {
     S.nodes = nodes;
+    S.wnodes = wnodes;
     S.grec = grec;
     __shared__ uint32_t lds[LdsLayout<K, SD>::kWords * kTraceBlock];
     __shared__ uint32_t wst[(kTraceBlock / 64) * kStackDeep];
@@ -59,8 +61,10 @@ __global__ void __launch_bounds__(kTraceBlock) k_wf_nee_setup(DeviceScene S, Fwd
 // two waves per SIMD without spills.
 template <int K, int SD>
 __global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(2)))
-k_wf_occlusion(DeviceScene S, wf::IO io, const float4* __restrict__ nodes, const float4* __restrict__ grec) {
+k_wf_occlusion(DeviceScene S, wf::IO io, const float4* __restrict__ nodes, const float4* __restrict__ wnodes,
+          const float4* __restrict__ grec) {
     S.nodes = nodes;
+    S.wnodes = wnodes;
     S.grec = grec;
     __shared__ uint32_t lds[LdsLayout<K, SD>::kWords * kTraceBlock];
     const Lane L{lds + threadIdx.x};
@@ -136,7 +140,7 @@ struct WfPrimary {
     hipStream_t s;
     template <int K, int SD>
     void run() const {
-        k_wf_primary<K, SD><<<S.num_blocks, kTraceBlock, 0, s>>>(S, out, io, S.nodes, S.grec);
+        k_wf_primary<K, SD><<<S.num_blocks, kTraceBlock, 0, s>>>(S, out, io, S.nodes, S.wnodes, S.grec);
     }
 };
 struct WfOcclusion {
@@ -145,7 +149,7 @@ struct WfOcclusion {
     hipStream_t s;
     template <int K, int SD>
     void run() const {
-        k_wf_occlusion<K, SD><<<wf::kBlocks, kTraceBlock, 0, s>>>(S, io, S.nodes, S.grec);
+        k_wf_occlusion<K, SD><<<wf::kBlocks, kTraceBlock, 0, s>>>(S, io, S.nodes, S.wnodes, S.grec);
     }
 };
 struct WfBounce {

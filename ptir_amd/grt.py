@@ -26,12 +26,12 @@ class GRTTracer:
                  particle_kernel_degree: float = 4.0, particle_kernel_min_response: float = 0.0113,
                  particle_kernel_density_clamping: bool = True, particle_radiance_sph_degree: int = 3,
                  enable_normals: bool = True, enable_hitcounts: bool = True, *, kbuffer: int = 16,
-                 traversal: str = "wave", builder: str = "lbvh"):
+                 traversal: str = "wave", builder: str = "lbvh", wide_walks: str = "none"):
         # the path tracer's handle: ptir_create and ptir_build_bvh are shared
         self._t = HipTracer(path, cuda_path, pipeline, backward_pipeline, primitive, particle_kernel_degree,
                             particle_kernel_min_response, particle_kernel_density_clamping,
                             particle_radiance_sph_degree, enable_normals, enable_hitcounts, False, False, False,
-                            kbuffer=kbuffer, traversal=traversal, builder=builder)
+                            kbuffer=kbuffer, traversal=traversal, builder=builder, wide_walks=wide_walks)
 
     def build_bvh(self, mog_pos, mog_rot, mog_scl, mog_dns, rebuild=True, allow_update=False):
         """OptixTracer::buildBVH (threedgrt_tracer/src/optixTracer.cpp:578-851)."""

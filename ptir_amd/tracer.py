@@ -82,6 +82,13 @@ class RaggedBatch:
         return out
 
 
+# which hierarchy walks use the 4-wide nodes (ptir_options.wide_walks): none, the wave-coherent
+# primary walk, the per-lane walks, or all of them
+WIDE_WALKS = {"none": 0, "wave": 1, "lane": 2, "all": 3}
+# forward schedules (ptir_options.schedule): one lane per path, or wavefront stages
+SCHEDULES = {"megakernel": 0, "wavefront": 1}
+
+
 class HipTracer:
     """Drop-in for the reference's `OptixTracer` pybind class."""
 
@@ -93,7 +100,7 @@ class HipTracer:
                  enable_metallic: bool = False, visualize_lights: bool = False, kbuffer: int = 16,
                  use_tape: bool = True, tape_hits_per_path: int = 64, traversal: str = "wave",
                  env_sampling: str = "alias", gradients: str = "ptir", schedule: str = "megakernel",
-                 builder: str = "lbvh"):
+                 builder: str = "lbvh", wide_walks: str = "none"):
         if pipeline != "reference" or backward_pipeline not in ("referenceBwd", "reference" + "Bwd"):
             raise ValueError(f"unsupported pipeline pair ({pipeline}, {backward_pipeline}); "
                              "only the PTIR 'reference' path tracer is built")
@@ -104,10 +111,12 @@ class HipTracer:
         if env_sampling not in ("alias", "cdf"):
             raise ValueError(f"env_sampling must be 'alias' (caller-built alias table) or 'cdf' "
                              f"(envmap.build_cdf_table), not {env_sampling!r}")
-        if schedule not in ("megakernel", "wavefront"):
+        if schedule not in SCHEDULES:
             raise ValueError(f"schedule must be 'megakernel' (one lane per path) or 'wavefront', not {schedule!r}")
         if builder not in ("lbvh", "ploc"):
             raise ValueError(f"builder must be 'lbvh' or 'ploc', not {builder!r}")
+        if wide_walks not in WIDE_WALKS:
+            raise ValueError(f"wide_walks must be one of {tuple(WIDE_WALKS)}, not {wide_walks!r}")
         if gradients not in ("ptir", "exact"):
             raise ValueError(f"gradients must be 'ptir' (the reference's backward) or 'exact', not {gradients!r}")
         # which backward the autograd function runs: the reference's path-replay backward
@@ -119,8 +128,8 @@ class HipTracer:
                          int(bool(particle_kernel_density_clamping)), int(particle_radiance_sph_degree),
                          int(bool(enable_normals)), int(bool(enable_hitcounts)), int(bool(enable_mis)),
                          int(bool(enable_metallic)), int(bool(visualize_lights)), int(kbuffer),
-                         {"lane": 0, "wave": 1}[traversal], {"megakernel": 0, "wavefront": 1}[schedule],
-                         {"lbvh": 0, "ploc": 1}[builder])
+                         {"lane": 0, "wave": 1}[traversal], SCHEDULES[schedule],
+                         {"lbvh": 0, "ploc": 1}[builder], WIDE_WALKS[wide_walks])
         h = C.c_void_p()
         L = _lib.lib()
         _lib.check(L.ptir_create(C.byref(o), C.byref(h)), None)
@@ -162,6 +171,13 @@ class HipTracer:
         """Raise PtirError (PTIR_ERR_BVH_DEPTH) when a traversal stack overflowed in any launch
         since the last check; synchronises the current stream.  ptir_check."""
         _lib.check(_lib.lib().ptir_check(self._h, _stream()), self._h)
+
+    def hierarchy_info(self) -> dict:
+        """The last built hierarchy: its 4-wide node count and the traversal stack entries its
+        walks can need (no counterpart in the reference).  ptir_hierarchy_info."""
+        nodes, need = C.c_int32(0), C.c_int32(0)
+        _lib.check(_lib.lib().ptir_hierarchy_info(self._h, C.byref(nodes), C.byref(need)), self._h)
+        return {"wide_nodes": int(nodes.value), "stack_need": int(need.value)}
 
     # -------------------------------------------------------------- build_bvh
     def build_bvh(self, mog_pos, mog_rot, mog_scl, mog_dns, rebuild=True, allow_update=False):
@@ -594,7 +610,8 @@ class Tracer:
             env_sampling=str(_conf_get(r, "env_sampling", "alias")),
             traversal=str(_conf_get(r, "primary_traversal", "wave")),
             schedule=str(_conf_get(r, "schedule", "megakernel")),
-            builder=str(_conf_get(r, "bvh_builder", "lbvh")))
+            builder=str(_conf_get(r, "bvh_builder", "lbvh")),
+            wide_walks=str(_conf_get(r, "wide_walks", "none")))
         # report a traversal-stack overflow after each render (one stream synchronisation;
         # skipped while a graph is being captured)
         self.check_errors = bool(_conf_get(r, "check_errors", True))

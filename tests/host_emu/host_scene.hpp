@@ -3,9 +3,10 @@
 //   meta.txt      "key value" lines (sizes and launch options)
 //   <name>.f32    float32 inputs: density [N,12], material [N,5], sh [N,S], snormal [N,3],
 //                 env [h,w,4], table [3,th,tw], ori/dir [B,H,W,3], ray_to_world [12]
-// The hierarchy is a median split with one Gaussian per leaf and the per-leaf records are
-// filled as k_records fills them (bvh.hip); the traversal returns the same hits for any
-// hierarchy.  The DeviceScene is filled as make_scene (api.cpp) fills it.
+// The hierarchy is a median split with one Gaussian per leaf, collapsed into the 4-wide nodes
+// the traversal walks by the phases of wide.hpp (as bvh.hip runs them), and the per-leaf
+// records are filled as k_records fills them (bvh.hip); the traversal returns the same hits
+// for any hierarchy.  The DeviceScene is filled as make_scene (api.cpp) fills it.
 #pragma once
 
 #include <cstdio>
@@ -93,7 +94,7 @@ struct Launch {
     int N = 0, B = 0, H = 0, W = 0, eh = 0, ew = 0, th = 0, tw = 0, shs = 0;
     size_t P = 0;
     std::vector<float> dens, mat, sh, sn, env, tab, ori, dir, T12, bounds;
-    std::vector<float4> nodes, grec, srec, prec;
+    std::vector<float4> nodes, wnodes, grec, srec, prec;
     std::vector<int> leaf_of;
     int error_flag = 0;
     DeviceScene S{};
@@ -182,6 +183,31 @@ struct Launch {
         };
         Box all;
         S.root = N > 1 ? split(0, N, all) : (leaf_of.push_back(0), all = box[0], ~0);
+        if (N > 1) {  // the 4-wide hierarchy: the phases of bvh.hip, in order
+            const int M = N - 1;
+            std::vector<int32_t> parent(M, -1), wmap(M);
+            std::vector<uint32_t> flag(M), pre(M);
+            for (int i = 0; i < M; ++i)
+                for (int c = 0; c < 2; ++c) {
+                    const int ch = __float_as_int(nodes[4 * size_t(i) + c].w);
+                    if (ch >= 0) parent[ch] = i;
+                }
+            for (int i = 0; i < M; ++i) flag[i] = wide::is_wide(parent.data(), i);
+            uint32_t run = 0;
+            for (int i = 0; i < M; ++i) {
+                pre[i] = run;
+                run += flag[i];
+            }
+            for (int i = 0; i < M; ++i) wide::number(flag.data(), pre.data(), i, wmap.data());
+            wnodes.assign(size_t(wide::kWords) * run, make_float4(NAN, NAN, NAN, NAN));
+            for (int i = 0; i < M; ++i) wide::fill(nodes.data(), wmap.data(), i, wnodes.data());
+            int need = 0;
+            for (int i = 0; i < M; ++i) need = std::max(need, wide::stack_need(parent.data(), wmap.data(), wnodes.data(), i));
+            if (need > kStackDeep) {
+                fprintf(stderr, "stack need %d exceeds %d\n", need, kStackDeep);
+                exit(2);
+            }
+        }
         bounds = {all.lo[0], all.lo[1], all.lo[2], all.hi[0], all.hi[1], all.hi[2]};
         grec.resize(4 * size_t(N));
         srec.resize(2 * size_t(N));
@@ -210,6 +236,8 @@ struct Launch {
         S.srec = srec.data();
         S.prec = prec.data();
         S.nodes = nodes.data();
+        S.wnodes = wnodes.data();
+        S.wide_walks = int(m.get("wide_walks", 3));
         S.bounds = bounds.data();
         S.sh = sh.data();
         S.env = reinterpret_cast<const float4*>(env.data());

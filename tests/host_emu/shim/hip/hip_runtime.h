@@ -13,6 +13,7 @@
 #define __forceinline__ inline
 #define __launch_bounds__(...)
 #define __builtin_amdgcn_readfirstlane(x) (x)
+#define __builtin_amdgcn_readlane(x, lane) (x)
 struct float4 { float x, y, z, w; };
 struct int4 { int x, y, z, w; };
 struct uint2 { unsigned x, y; };
@@ -27,6 +28,7 @@ inline hipError_t hipGetLastError() { return 0; }
 inline unsigned long long __ballot(int p) { return p ? 1ull : 0ull; }
 inline int __ffsll(long long x) { return __builtin_ffsll(x); }
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
+inline int __popc(unsigned x) { return __builtin_popcount(x); }
 inline unsigned __lane_id() { return 0; }
 template <class T> inline T __shfl(T v, int) { return v; }
 inline float __uint_as_float(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }

@@ -70,7 +70,9 @@ int main(int argc, char** argv) {
     // the wavefront stages, path by path
     // (each stage saves the path's state to a slot and the next loads it back: here slot = path)
     std::vector<float> state(size_t(wf::kWords) * P, 0.f);
-    wf::IO io{state.data(), state.data(), nullptr, nullptr, nullptr, nullptr, nullptr, P};
+    wf::IO io{};
+    io.sa = io.sb = state.data();
+    io.P = P;
     size_t staged = 0, jobs = 0;
     la.for_each_path([&](const PixelId& px, int t) {
         const Lane L{lds.data() + t};

@@ -25,13 +25,13 @@ def test_autotune_returns_the_fastest_and_the_same_image(device):
     print("autotune small scene", flush=True)
     res = autotune(geom, args, sc.frame, g.ray_to_world, g.rays_ori, g.rays_dir,
                    tracer_kwargs=dict(enable_metallic=True), trace_kwargs=kw, backward=True, reps=1)
-    assert len(res.timings) == 24
+    assert len(res.timings) == 48
     assert all(t > 0 for t in res.timings.values())
     assert res.ms == min(res.timings.values())
-    assert res.timings[(res.kbuffer, res.traversal, res.schedule, res.builder)] == res.ms
+    assert res.timings[(res.kbuffer, res.traversal, res.schedule, res.builder, res.wide_walks)] == res.ms
     d = res.as_dict()
-    assert set(d) == {"kbuffer", "traversal", "schedule", "builder", "ms", "timings_ms"}
-    assert len(d["timings_ms"]) == 24
+    assert set(d) == {"kbuffer", "traversal", "schedule", "builder", "wide_walks", "ms", "timings_ms"}
+    assert len(d["timings_ms"]) == 48
 
     outs = []
     for tkw in (dict(), res.tracer_kwargs()):

@@ -41,6 +41,11 @@ def test_ploc_traces_what_the_lbvh_traces(device, which):
         tr.build_bvh(g.positions, g.rotation, g.scale, g.density)
         torch.cuda.synchronize()
         print(f"{which} {builder}: build {1e3 * (time.perf_counter() - t0):.2f} ms", flush=True)
+        info = tr.hierarchy_info()
+        print(f"{which} {builder}: {info}", flush=True)
+        # the 4-wide nodes are the even-depth binary nodes: at least a third of the n - 1
+        assert (sc.num_gaussians - 1) / 3 <= info["wide_nodes"] < sc.num_gaussians
+        assert 0 <= info["stack_need"] <= 64
         res[builder] = _trace(tr, g, args, sc, counters=True)
         cnt[builder] = tr.last_counters
         tr.check()
@@ -112,3 +117,28 @@ def test_ploc_tiny_and_duplicate_scenes(device, n):
     torch.cuda.synchronize()
     for i, (a, b) in enumerate(zip(*outs)):
         assert torch.equal(a, b), f"output {i}"
+
+
+@pytest.mark.parametrize("which", ["small", "object"])
+def test_wide_walks_trace_what_the_binary_walks_trace(device, which):
+    """The 4-wide nodes (collapsed from the same binary tree) under the wave-coherent walk, the
+    per-lane walks, or both: every output equal bit for bit, fewer node visits per traversal,
+    both schedules."""
+    sc = scenes.small_scene(5, n=4096, res=64, spp=1, max_bounces=3) if which == "small" else scenes.object_scene(0)
+    g, args = _setup(sc, device)
+    for schedule in ("megakernel", "wavefront"):
+        res, cnt = {}, {}
+        for ww in ("none", "wave", "lane", "all"):
+            tr = HipTracer(enable_metallic=True, builder="ploc", schedule=schedule, wide_walks=ww)
+            tr.build_bvh(g.positions, g.rotation, g.scale, g.density)
+            res[ww] = _trace(tr, g, args, sc, counters=True)
+            cnt[ww] = tr.last_counters
+            tr.check()
+            print(f"{which} {schedule} wide={ww}: {cnt[ww]}", flush=True)
+        for ww in ("wave", "lane", "all"):
+            for i, (a, b) in enumerate(zip(res["none"], res[ww])):
+                assert torch.equal(a, b), f"{schedule} wide={ww} output {i}"
+            for k in ("paths", "segments", "candidates", "accepted", "accepted_primary", "shadow_candidates",
+                      "traversals"):
+                assert cnt["none"][k] == cnt[ww][k], (ww, k)
+        assert cnt["all"]["nodes"] < 0.7 * cnt["none"]["nodes"]

@@ -4,7 +4,7 @@ per-path body (fwd_path, trace_path.hpp), both compiled for the host and run one
 turn must write exactly what the one-lane body writes -- all 13 outputs, visibility, radiance-
 cache records and the path tape (header, records, hit lists) -- bit for bit, under MIS on and
 off, INDIRECT, Russian roulette, both cache modes, tapes whose hits spill into pool chunks, and
-per-wave and per-lane primary traversal."""
+per-wave and per-lane primary traversal, over the binary and the 4-wide nodes."""
 
 import pytest
 
@@ -13,10 +13,12 @@ from gpu_helpers import env_table
 from ptir_amd import scenes
 
 CASES = {
-    "mis_tape": dict(max_bounces=3, enable_mis=True, tape=1, tape_cap=16, wave_traversal=1),
-    "no_mis": dict(max_bounces=3, enable_mis=False, tape=1, tape_cap=64, wave_traversal=0),
-    "indirect": dict(max_bounces=2, enable_mis=True, render_opts=1, tape=1, tape_cap=8, wave_traversal=1),
-    "roulette_mb5": dict(max_bounces=5, enable_mis=True, rr_start=1, tape=1, tape_cap=32, wave_traversal=1),
+    "mis_tape": dict(max_bounces=3, enable_mis=True, tape=1, tape_cap=16, wave_traversal=1, wide_walks=3),
+    "no_mis": dict(max_bounces=3, enable_mis=False, tape=1, tape_cap=64, wave_traversal=0, wide_walks=0),
+    "indirect": dict(max_bounces=2, enable_mis=True, render_opts=1, tape=1, tape_cap=8, wave_traversal=1,
+                     wide_walks=1),
+    "roulette_mb5": dict(max_bounces=5, enable_mis=True, rr_start=1, tape=1, tape_cap=32, wave_traversal=1,
+                         wide_walks=2),
     "cache_query": dict(max_bounces=4, enable_mis=True, cache_mode=1, tape=1, tape_cap=64, wave_traversal=1),
     "cache_train": dict(max_bounces=4, enable_mis=True, cache_mode=2, tape=0, wave_traversal=0),
     "one_vertex": dict(max_bounces=1, enable_mis=True, tape=1, tape_cap=64, wave_traversal=1),
