@@ -29,13 +29,9 @@ k_trace_fwd(DeviceScene S, FwdOut out,
     fwd_path<K, SD>(S, out, L, wstack, px);
 }
 
-// referenceBwdOptix.cu:27-74.  TAPE = false replays the path by re-tracing every
-// segment; TAPE = true reads the segments and their hits from the forward's tape.
+// One path of the backward (the kernel below).
 template <int K, int SD, bool TAPE>
-__global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(waves_per_eu(K, TAPE ? 0 : SD))))
-k_trace_bwd(DeviceScene S, BwdIO io) {
-    __shared__ uint32_t lds[(TAPE ? 1 : LdsLayout<K, SD>::kWords) * kTraceBlock];
-    const Lane L{lds + threadIdx.x};
+__device__ __forceinline__ void bwd_path(const DeviceScene& S, const BwdIO& io, const Lane& L, MatTable* tab) {
     const PixelId px = pixel_of(S);
     if (!px.ok) return;
     const size_t pix = px.pix;
@@ -53,7 +49,7 @@ k_trace_bwd(DeviceScene S, BwdIO io) {
     } else {
         if (io.replay_mode == 1 && *io.tape_state == 0 && !(prec[0] & kTapeOverflow)) return;
     }
-    const Flush f{&io, pix, paths, &L, prec ? prec[3] : kNoChunk};
+    const Flush f{&io, pix, paths, &L, prec ? prec[3] : kNoChunk, tab};
     Rng rng;
     rng.seed(Rng::launch_seed(uint32_t(px.x), uint32_t(px.y), uint32_t(px.z), S.frame));
     Path p;
@@ -111,6 +107,24 @@ k_trace_bwd(DeviceScene S, BwdIO io) {
             none.hits = nullptr;
             ray_intersect<K, SD>(S, p.cur.ray, p.cur, rng, true, false, false, nullptr, L, none);
         }
+    }
+}
+
+// referenceBwdOptix.cu:27-74.  TAPE = false replays the path by re-tracing every
+// segment; TAPE = true reads the segments and their hits from the forward's tape and gathers
+// the workgroup's material gradients in LDS (MatTable) before they reach memory.
+template <int K, int SD, bool TAPE>
+__global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(waves_per_eu(K, TAPE ? 0 : SD))))
+k_trace_bwd(DeviceScene S, BwdIO io) {
+    __shared__ uint32_t lds[(TAPE ? 1 : LdsLayout<K, SD>::kWords) * kTraceBlock];
+    const Lane L{lds + threadIdx.x};
+    if constexpr (TAPE) {
+        __shared__ MatTable tab;
+        mat_table_clear(tab);
+        bwd_path<K, SD, TAPE>(S, io, L, &tab);
+        mat_table_flush(tab, io.dmat_leaf);
+    } else {
+        bwd_path<K, SD, TAPE>(S, io, L, nullptr);
     }
 }
 

@@ -1209,11 +1209,67 @@ __device__ __forceinline__ void sample_next(const DeviceScene& S, Path& p, Rng& 
 // Where the backward's material-gradient flushes come from: a re-trace of the segment
 // (pathTracer.cuh:313-390) or, with a tape, the segment's recorded hits.  Both give the
 // same per-hit products w * (g / opacity), with w = alpha * T computed identically.
+// Material gradients of one workgroup gathered in LDS before they reach memory: an open-
+// addressing table from leaf slot to the five gradient sums.  The primary rays of a 16 x 8 pixel
+// tile hit far fewer distinct Gaussians than hits (a Gaussian covers many neighbouring pixels),
+// so one global atomic per distinct Gaussian and float replaces one per hit.  A hit whose slot
+// finds no room within kMatProbe probes goes to global memory directly.
+constexpr int kMatTable = 1024;  // entries (power of two)
+constexpr int kMatProbe = 16;
+constexpr uint32_t kMatEmpty = 0xffffffffu;
+struct MatTable {
+    uint32_t key[kMatTable];
+    float val[5][kMatTable];
+};
+
+// Every thread of the workgroup calls it (it synchronises the workgroup).
+__device__ __forceinline__ void mat_table_clear(MatTable& t) {
+    for (int i = threadIdx.x; i < kMatTable; i += blockDim.x) {
+        t.key[i] = kMatEmpty;
+#pragma unroll
+        for (int c = 0; c < 5; ++c) t.val[c][i] = 0.f;
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void mat_add(MatTable* t, float* dmat_leaf, uint32_t slot, const float v[5], int nv) {
+    if (t) {
+        uint32_t h = (slot * 2654435761u) >> (32 - 10);
+        static_assert(kMatTable == 1 << 10, "hash width");
+        for (int k = 0; k < kMatProbe; ++k, h = (h + 1u) & uint32_t(kMatTable - 1)) {
+            uint32_t cur = t->key[h];
+            if (cur == kMatEmpty) cur = atomicCAS(&t->key[h], kMatEmpty, slot);
+            if (cur == kMatEmpty || cur == slot) {
+                for (int c = 0; c < nv; ++c) atomicAdd(&t->val[c][h], v[c]);
+                return;
+            }
+        }
+    }
+    float* q = dmat_leaf + 5 * size_t(slot);
+    for (int c = 0; c < nv; ++c) atomicAdd(q + c, v[c]);
+}
+
+// Every thread of the workgroup calls it after its last mat_add (it synchronises the workgroup).
+__device__ __forceinline__ void mat_table_flush(MatTable& t, float* dmat_leaf) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < kMatTable; i += blockDim.x) {
+        const uint32_t slot = t.key[i];
+        if (slot == kMatEmpty) continue;
+        float* q = dmat_leaf + 5 * size_t(slot);
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            const float v = t.val[c][i];
+            if (v != 0.f) atomicAdd(q + c, v);
+        }
+    }
+}
+
 struct Flush {
     const BwdIO* io;
     size_t pix, paths;
     const Lane* L;
     uint32_t chunk;  // the path's tape pool chunk (kNoChunk: none)
+    MatTable* tab;   // the workgroup's gradient table, or null (atomics straight to memory)
 };
 
 __device__ __forceinline__ void tape_flush(const DeviceScene& S, const Flush& f, uint32_t off, uint32_t cnt, float opacity,
@@ -1225,15 +1281,12 @@ __device__ __forceinline__ void tape_flush(const DeviceScene& S, const Flush& f,
     const Tape& tape = f.io->tape;
     uint2* own = tape.hits(f.pix, f.paths);
     uint2* pl = tape.pool(f.paths);
+    const int nv = S.enable_metallic ? 5 : 4;
     for (uint32_t i = off; i < off + cnt; ++i) {
         const uint2 e = *tape.entry(own, pl, f.chunk, i);
         const float w = __uint_as_float(e.y);
-        float* q = f.io->dmat_leaf + 5 * size_t(e.x);
-        atomicAdd(q + 0, w * ga.x);
-        atomicAdd(q + 1, w * ga.y);
-        atomicAdd(q + 2, w * ga.z);
-        atomicAdd(q + 3, w * gr);
-        if (S.enable_metallic) atomicAdd(q + 4, w * gm);
+        const float v[5] = {w * ga.x, w * ga.y, w * ga.z, w * gr, w * gm};
+        mat_add(f.tab, f.io->dmat_leaf, e.x, v, nv);
         count(S, C_MAT_ATOMICS);
     }
 }
