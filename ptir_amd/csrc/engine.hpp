@@ -76,7 +76,7 @@ struct Walk {
         for (;;) {
             if (__ballot(walk && q0 < 0) == 0) break;  // every walking lane has a leaf queued
             if (wide_lane && walk && q0 < 0) {
-                const float4* nd = S.wnodes + wide::kWords * link;
+                const wide::Node4 nd = wide::load4(S.wnodes + wide::kWords * link);
                 count(S, C_NODES);
                 float tn[4];
                 int32_t cl[4];

@@ -127,7 +127,7 @@ __device__ __forceinline__ int collect(const DeviceScene& S, const Ray& r, float
             for (;;) {
                 if (__ballot(walk && q0 < 0) == 0) break;  // every walking lane has a leaf queued
                 if (walk && q4 < 0) {  // room for four leaves
-                    const float4* nd = S.wnodes + wide::kWords * link;
+                    const wide::Node4 nd = wide::load4(S.wnodes + wide::kWords * link);
                     count(S, C_NODES);
                     float tn[4];
                     int32_t cl[4];
@@ -781,7 +781,7 @@ __device__ __forceinline__ int wave_collect(const DeviceScene& S, const Ray& r, 
         int sp = 0;         // wave-uniform
         for (;;) {
             link = __builtin_amdgcn_readfirstlane(link);
-            const float4* nd = S.wnodes + wide::kWords * link;
+            const wide::Node4 nd = wide::load4(S.wnodes + wide::kWords * link);
             if (need) count(S, C_NODES);
             float tn[4];
             int32_t cl[4];

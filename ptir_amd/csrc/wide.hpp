@@ -96,11 +96,19 @@ __host__ __device__ inline int stack_need(const int32_t* parent, const int32_t* 
     return need;
 }
 
+// A node's boxes and links, loaded together (one wait for the whole record).
+struct Node4 {
+    float4 lx, ly, lz, hx, hy, hz, links;
+};
+__device__ __forceinline__ Node4 load4(const float4* nd) {
+    return Node4{nd[0], nd[1], nd[2], nd[3], nd[4], nd[5], nd[6]};
+}
+
 // Slab test of the four children of a node against a ray (inverse direction inv) in
 // (tmin, cut): bit c of the result is set when the ray enters child c's box before cut;
 // tn[c] is the entry distance.
-__device__ __forceinline__ uint32_t test4(const float4* nd, V3 o, V3 inv, float tmin, float cut, float tn[4]) {
-    const float4 lx = nd[0], ly = nd[1], lz = nd[2], hx = nd[3], hy = nd[4], hz = nd[5];
+__device__ __forceinline__ uint32_t test4(const Node4& nd, V3 o, V3 inv, float tmin, float cut, float tn[4]) {
+    const float4 lx = nd.lx, ly = nd.ly, lz = nd.lz, hx = nd.hx, hy = nd.hy, hz = nd.hz;
     const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, lya[4] = {ly.x, ly.y, ly.z, ly.w}, lza[4] = {lz.x, lz.y, lz.z, lz.w};
     const float hxa[4] = {hx.x, hx.y, hx.z, hx.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
     uint32_t m = 0u;
@@ -117,15 +125,15 @@ __device__ __forceinline__ uint32_t test4(const float4* nd, V3 o, V3 inv, float 
     return m;
 }
 
-__device__ __forceinline__ void links4(const float4* nd, int32_t l[4]) {
-    const float4 q = nd[6];
+__device__ __forceinline__ void links4(const Node4& nd, int32_t l[4]) {
+    const float4 q = nd.links;
     l[0] = __float_as_int(q.x);
     l[1] = __float_as_int(q.y);
     l[2] = __float_as_int(q.z);
     l[3] = __float_as_int(q.w);
 }
 
-// Sorts (key, link) pairs ascending by key (ties keep the lower child first): a five-comparator
+// Sorts (key, link) pairs ascending by key (equal keys in no particular order): a five-comparator
 // network over four entries.
 __device__ __forceinline__ void sort4(float k[4], int32_t l[4]) {
     auto cs = [&](int a, int b) {

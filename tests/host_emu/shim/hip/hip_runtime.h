@@ -39,6 +39,8 @@ inline float atomicAdd(float* p, float v) { float o = *p; *p = o + v; return o; 
 inline unsigned long long atomicAdd(unsigned long long* p, unsigned long long v) { auto o = *p; *p = o + v; return o; }
 inline uint32_t atomicAdd(uint32_t* p, uint32_t v) { auto o = *p; *p = o + v; return o; }
 inline int atomicOr(int* p, int v) { int o = *p; *p = o | v; return o; }
+inline uint32_t atomicCAS(uint32_t* p, uint32_t cmp, uint32_t v) { uint32_t o = *p; if (o == cmp) *p = v; return o; }
+inline void __syncthreads() {}
 inline float rsqrtf(float x) { return 1.0f / std::sqrt(x); }
 inline int min(int a, int b) { return a < b ? a : b; }
 inline int max(int a, int b) { return a > b ? a : b; }
