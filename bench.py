@@ -344,7 +344,7 @@ def main() -> int:
     kw = dict(render_opts=0, sph_degree=scene.sph_degree, min_transmittance=scene.min_transmittance,
               max_bounces=scene.max_bounces)
     tuned = None
-    tcfg = dict(kbuffer=16, traversal="wave", schedule="megakernel", builder="ploc", wide_walks="none")
+    tcfg = dict(kbuffer=16, traversal="wave", schedule="megakernel", builder="ploc", wide_walks="all")
     if not args.no_autotune:
         print("autotuning the object-scale step", file=sys.stderr, flush=True)
         tuned = autotune((g.positions, g.rotation, g.scale, g.density), fargs[4:], scene.frame, g.ray_to_world,
