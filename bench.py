@@ -304,6 +304,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tape", action="store_true", help="backward re-traces every segment (reference behaviour)")
     ap.add_argument("--no-nrc", action="store_true", help="skip the radiance-cache comparison")
+    ap.add_argument("--primary-only", action="store_true",
+                    help="time only the 3DGRT primary-only tracer on the object-scale rays (for profiling)")
     ap.add_argument("--no-workloads", action="store_true",
                     help="skip the batched (ragged, 500k) and scene-scale (3M) inversion-step timings")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -319,6 +321,9 @@ def main() -> int:
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": n * 4, "copies": 5}))
+        return 0
+    if args.primary_only:
+        print(json.dumps({"primary_only": time_primary_only(steps=args.steps, warmup=args.warmup)}))
         return 0
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
