@@ -37,7 +37,7 @@ struct Walk {
             bs[j] = kInvalid;
         }
         count(S, C_TRAVERSALS);
-        inv = v3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
+        inv = slab_inv(r.d);
         tmin = lo;
         tmax = hi;
         q0 = q1 = q2 = q3 = -1;

@@ -51,17 +51,28 @@ __device__ __forceinline__ bool cand_test(const DeviceScene& S, int slot, const 
     const V3 gro = v3(dot(xyz(a0), dp), dot(xyz(a1), dp), dot(xyz(a2), dp));
     const V3 grdu = v3(dot(xyz(a0), r.d), dot(xyz(a1), r.d), dot(xyz(a2), r.d));
     const float dd = dot(grdu, grdu);
-    t = -dot(gro, grdu) / dd;
+    // the hardware reciprocal (1 ulp) instead of an IEEE division: every path, schedule and the
+    // backward's re-trace compute t* here, so they agree bit for bit
+    t = -dot(gro, grdu) * __builtin_amdgcn_rcpf(dd);
     if (!(t > tmin && t < tmax)) return false;
     const V3 q = gro + grdu * t;
     if (fmaxf(fabsf(q.x), fmaxf(fabsf(q.y), fabsf(q.z))) > p.y) return false;
     return dot(q, q) < 9.0f * dd;
 }
 
+// Reciprocal of a ray direction for the slab tests: a zero (or denormal) component becomes
+// +-1e20, which leaves that axis unconstrained for an origin between the planes and empty
+// outside, and keeps lo * inv - o * inv finite, so each plane distance is one fused multiply-add.
+__device__ __forceinline__ V3 slab_inv(V3 d) {
+    auto one = [](float c) { return fabsf(c) > 1e-20f ? 1.0f / c : copysignf(1e20f, c); };
+    return v3(one(d.x), one(d.y), one(d.z));
+}
+
 __device__ __forceinline__ bool slab(float4 lo, float4 hi, const Ray& r, V3 inv, float tmin, float tmax, float& tn) {
-    const float ax = (lo.x - r.o.x) * inv.x, bx = (hi.x - r.o.x) * inv.x;
-    const float ay = (lo.y - r.o.y) * inv.y, by = (hi.y - r.o.y) * inv.y;
-    const float az = (lo.z - r.o.z) * inv.z, bz = (hi.z - r.o.z) * inv.z;
+    const V3 oi = r.o * inv;
+    const float ax = fmaf(lo.x, inv.x, -oi.x), bx = fmaf(hi.x, inv.x, -oi.x);
+    const float ay = fmaf(lo.y, inv.y, -oi.y), by = fmaf(hi.y, inv.y, -oi.y);
+    const float az = fmaf(lo.z, inv.z, -oi.z), bz = fmaf(hi.z, inv.z, -oi.z);
     tn = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fminf(az, bz));
     const float tf = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
     return tn <= tf && tf >= tmin && tn <= tmax;
@@ -98,7 +109,7 @@ __device__ __forceinline__ int collect(const DeviceScene& S, const Ray& r, float
             insert(t, uint32_t(slot));
     };
     count(S, C_TRAVERSALS);
-    const V3 inv = v3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
+    const V3 inv = slab_inv(r.d);
     if (S.n > 0 && S.root < 0) {
         leaf(~S.root);
     } else if (S.n > 0 && (S.wide_walks & 2)) {
@@ -772,7 +783,7 @@ __device__ __forceinline__ int wave_collect(const DeviceScene& S, const Ray& r, 
         if (__ballot(ins) != 0) insert(ins, t, s);
     };
     if (need) count(S, C_TRAVERSALS);
-    const V3 inv = v3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
+    const V3 inv = slab_inv(r.d);
     if (S.n == 0) need = false;
     if (S.n > 0 && S.root < 0) {
         leaf(need, ~S.root);

@@ -104,19 +104,21 @@ __device__ __forceinline__ Node4 load4(const float4* nd) {
     return Node4{nd[0], nd[1], nd[2], nd[3], nd[4], nd[5], nd[6]};
 }
 
-// Slab test of the four children of a node against a ray (inverse direction inv) in
+// Slab test of the four children of a node against a ray (inverse direction inv, finite: see
+// slab_inv) in
 // (tmin, cut): bit c of the result is set when the ray enters child c's box before cut;
 // tn[c] is the entry distance.
 __device__ __forceinline__ uint32_t test4(const Node4& nd, V3 o, V3 inv, float tmin, float cut, float tn[4]) {
     const float4 lx = nd.lx, ly = nd.ly, lz = nd.lz, hx = nd.hx, hy = nd.hy, hz = nd.hz;
     const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, lya[4] = {ly.x, ly.y, ly.z, ly.w}, lza[4] = {lz.x, lz.y, lz.z, lz.w};
     const float hxa[4] = {hx.x, hx.y, hx.z, hx.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+    const float oix = o.x * inv.x, oiy = o.y * inv.y, oiz = o.z * inv.z;
     uint32_t m = 0u;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        const float ax = (lxa[c] - o.x) * inv.x, bx = (hxa[c] - o.x) * inv.x;
-        const float ay = (lya[c] - o.y) * inv.y, by = (hya[c] - o.y) * inv.y;
-        const float az = (lza[c] - o.z) * inv.z, bz = (hza[c] - o.z) * inv.z;
+        const float ax = fmaf(lxa[c], inv.x, -oix), bx = fmaf(hxa[c], inv.x, -oix);
+        const float ay = fmaf(lya[c], inv.y, -oiy), by = fmaf(hya[c], inv.y, -oiy);
+        const float az = fmaf(lza[c], inv.z, -oiz), bz = fmaf(hza[c], inv.z, -oiz);
         const float n = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fminf(az, bz));
         const float f = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
         tn[c] = n;

@@ -15,6 +15,7 @@
 #define __builtin_amdgcn_readfirstlane(x) (x)
 #define __builtin_amdgcn_readlane(x, lane) (x)
 #define __builtin_amdgcn_fence(order, scope) ((void)0)
+#define __builtin_amdgcn_rcpf(x) (1.0f / (x))
 #define __align__(n) alignas(n)
 struct float4 { float x, y, z, w; };
 struct int4 { int x, y, z, w; };
