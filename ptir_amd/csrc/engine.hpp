@@ -17,8 +17,7 @@ namespace eng {
 
 template <int K, int SD>
 struct Walk {
-    float bt[K];
-    uint32_t bs[K];
+    uint64_t kb[K];  // K-buffer keys (kb_key)
     V3 inv;
     float tmin, tmax;
     int link, sp, q0, q1, q2, q3;
@@ -32,10 +31,7 @@ struct Walk {
     // A new walk of ray r in (lo, hi).
     __device__ __forceinline__ void start(const DeviceScene& S, const Ray& r, float lo, float hi) {
 #pragma unroll
-        for (int j = 0; j < K; ++j) {
-            bt[j] = __int_as_float(0x7f800000);
-            bs[j] = kInvalid;
-        }
+        for (int j = 0; j < K; ++j) kb[j] = kKbEmpty;
         count(S, C_TRAVERSALS);
         inv = slab_inv(r.d);
         tmin = lo;
@@ -56,16 +52,13 @@ struct Walk {
         q0 = s;
     }
 
-    __device__ __forceinline__ void insert(float t, uint32_t s) {
+    __device__ __forceinline__ void insert(uint64_t k) {
 #pragma unroll
         for (int j = 0; j < K; ++j) {
-            const bool lt = (t < bt[j]) || (t == bt[j] && s < bs[j]);
-            const float tt = lt ? bt[j] : t;
-            const uint32_t ss = lt ? bs[j] : s;
-            bt[j] = lt ? t : bt[j];
-            bs[j] = lt ? s : bs[j];
-            t = tt;
-            s = ss;
+            const bool lt = k < kb[j];
+            const uint64_t kk = lt ? kb[j] : k;
+            kb[j] = lt ? k : kb[j];
+            k = kk;
         }
     }
 
@@ -80,7 +73,7 @@ struct Walk {
                 count(S, C_NODES);
                 float tn[4];
                 int32_t cl[4];
-                const uint32_t m = wide::test4(nd, r.o, inv, tmin, fminf(tmax, bt[K - 1]), tn);
+                const uint32_t m = wide::test4(nd, r.o, inv, tmin, fminf(tmax, kb_t(kb[K - 1])), tn);
                 wide::links4(nd, cl);
                 int ni = 0;
 #pragma unroll
@@ -110,7 +103,7 @@ struct Walk {
                 const float4* nd = S.nodes + 4 * link;
                 const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2], n3 = nd[3];
                 count(S, C_NODES);
-                const float cut = fminf(tmax, bt[K - 1]);
+                const float cut = fminf(tmax, kb_t(kb[K - 1]));
                 float tl, tr;
                 bool hl = slab(n0, n1, r, inv, tmin, cut, tl);
                 bool hr = slab(n2, n3, r, inv, tmin, cut, tr);
@@ -146,9 +139,10 @@ struct Walk {
                 q3 = -1;
                 float t;
                 count(S, C_LEAF_TESTS);
-                if (cand_test(S, slot, r, tmin, tmax, t) &&
-                    (t < bt[K - 1] || (t == bt[K - 1] && uint32_t(slot) < bs[K - 1])))
-                    insert(t, uint32_t(slot));
+                if (cand_test(S, slot, r, tmin, tmax, t)) {
+                    const uint64_t k = kb_key(t, uint32_t(slot));
+                    if (k < kb[K - 1]) insert(k);
+                }
             }
         }
     }
@@ -235,9 +229,9 @@ __device__ __forceinline__ void occlusion(const DeviceScene& S, ShadowJob* jobs,
             int cnt = 0;
 #pragma unroll
             for (int k = 0; k < K; ++k) {
-                L.at(k) = __float_as_uint(w.bt[k]);
-                L.at(K + k) = w.bs[k];
-                cnt += w.bs[k] != kInvalid ? 1 : 0;
+                L.at(k) = uint32_t(w.kb[k] >> 32);
+                L.at(K + k) = kb_s(w.kb[k]);
+                cnt += kb_s(w.kb[k]) != kInvalid ? 1 : 0;
             }
             float batchEnd = startT;
             bool occ = false;

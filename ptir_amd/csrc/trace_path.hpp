@@ -78,35 +78,38 @@ __device__ __forceinline__ bool slab(float4 lo, float4 hi, const Ray& r, V3 inv,
     return tn <= tf && tf >= tmin && tn <= tmax;
 }
 
+// K-buffer entries as 64-bit keys, the distance's bits above the leaf slot: candidate
+// distances are positive (t > tmin > 0), where the bits order like the floats, so one
+// unsigned compare orders entries by (t, slot).  An empty entry is (+inf, kInvalid).
+__device__ __forceinline__ uint64_t kb_key(float t, uint32_t s) { return (uint64_t(__float_as_uint(t)) << 32) | s; }
+__device__ __forceinline__ float kb_t(uint64_t k) { return __uint_as_float(uint32_t(k >> 32)); }
+__device__ __forceinline__ uint32_t kb_s(uint64_t k) { return uint32_t(k); }
+constexpr uint64_t kKbEmpty = (uint64_t(0x7f800000u) << 32) | uint64_t(kInvalid);
+
 // Collect the K closest candidates in (tmin, tmax).  Returns the count; the sorted
 // entries are left in LDS: t at words [0,K), leaf slot at words [K,2K).
 template <int K, int SD>
 __device__ __forceinline__ int collect(const DeviceScene& S, const Ray& r, float tmin, float tmax, const Lane& L) {
-    float bt[K];
-    uint32_t bs[K];
+    uint64_t kb[K];
 #pragma unroll
-    for (int j = 0; j < K; ++j) {
-        bt[j] = __int_as_float(0x7f800000);
-        bs[j] = kInvalid;
-    }
+    for (int j = 0; j < K; ++j) kb[j] = kKbEmpty;
     unsigned long long ncand = 0;
-    auto insert = [&](float t, uint32_t s) {
+    auto insert = [&](uint64_t k) {
 #pragma unroll
         for (int j = 0; j < K; ++j) {
-            const bool lt = (t < bt[j]) || (t == bt[j] && s < bs[j]);
-            const float tt = lt ? bt[j] : t;
-            const uint32_t ss = lt ? bs[j] : s;
-            bt[j] = lt ? t : bt[j];
-            bs[j] = lt ? s : bs[j];
-            t = tt;
-            s = ss;
+            const bool lt = k < kb[j];
+            const uint64_t kk = lt ? kb[j] : k;
+            kb[j] = lt ? k : kb[j];
+            k = kk;
         }
     };
     auto leaf = [&](int slot) {
         float t;
         count(S, C_LEAF_TESTS);
-        if (cand_test(S, slot, r, tmin, tmax, t) && (t < bt[K - 1] || (t == bt[K - 1] && uint32_t(slot) < bs[K - 1])))
-            insert(t, uint32_t(slot));
+        if (cand_test(S, slot, r, tmin, tmax, t)) {
+            const uint64_t k = kb_key(t, uint32_t(slot));
+            if (k < kb[K - 1]) insert(k);
+        }
     };
     count(S, C_TRAVERSALS);
     const V3 inv = slab_inv(r.d);
@@ -142,7 +145,7 @@ __device__ __forceinline__ int collect(const DeviceScene& S, const Ray& r, float
                     count(S, C_NODES);
                     float tn[4];
                     int32_t cl[4];
-                    const uint32_t m = wide::test4(nd, r.o, inv, tmin, fminf(tmax, bt[K - 1]), tn);
+                    const uint32_t m = wide::test4(nd, r.o, inv, tmin, fminf(tmax, kb_t(kb[K - 1])), tn);
                     wide::links4(nd, cl);
                     int ni = 0;
 #pragma unroll
@@ -206,7 +209,7 @@ __device__ __forceinline__ int collect(const DeviceScene& S, const Ray& r, float
                     const float4* nd = S.nodes + 4 * link;
                     const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2], n3 = nd[3];
                     count(S, C_NODES);
-                    const float cut = fminf(tmax, bt[K - 1]);
+                    const float cut = fminf(tmax, kb_t(kb[K - 1]));
                     float tl, tr;
                     bool hl = slab(n0, n1, r, inv, tmin, cut, tl);
                     bool hr = slab(n2, n3, r, inv, tmin, cut, tr);
@@ -250,9 +253,9 @@ __device__ __forceinline__ int collect(const DeviceScene& S, const Ray& r, float
     int cnt = 0;
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-        L.at(j) = __float_as_uint(bt[j]);
-        L.at(K + j) = bs[j];
-        cnt += bs[j] != kInvalid ? 1 : 0;
+        L.at(j) = uint32_t(kb[j] >> 32);
+        L.at(K + j) = kb_s(kb[j]);
+        cnt += kb_s(kb[j]) != kInvalid ? 1 : 0;
     }
     return cnt;
 }
@@ -751,26 +754,20 @@ struct SegTape {
 template <int K, int SD>
 __device__ __forceinline__ int wave_collect(const DeviceScene& S, const Ray& r, float tmin, float tmax, bool need,
                                             const Lane& L, uint32_t* wstack) {
-    float bt[K];
-    uint32_t bs[K];
+    uint64_t kb[K];
 #pragma unroll
-    for (int j = 0; j < K; ++j) {
-        bt[j] = __int_as_float(0x7f800000);
-        bs[j] = kInvalid;
-    }
+    for (int j = 0; j < K; ++j) kb[j] = kKbEmpty;
     // branch-free sorted insertion (all lanes run it together); ok = false leaves it unchanged
-    auto insert = [&](bool ok, float t, uint32_t s) {
-        auto below = [&](int j) { return ok & ((t < bt[j]) | ((t == bt[j]) & (s < bs[j]))); };
+    auto insert = [&](bool ok, uint64_t k) {
+        auto below = [&](int j) { return ok & (k < kb[j]); };
         bool bj = below(K - 1);
 #pragma unroll
         for (int j = K - 1; j >= 1; --j) {
             const bool bp = below(j - 1);
-            bt[j] = bp ? bt[j - 1] : (bj ? t : bt[j]);
-            bs[j] = bp ? bs[j - 1] : (bj ? s : bs[j]);
+            kb[j] = bp ? kb[j - 1] : (bj ? k : kb[j]);
             bj = bp;
         }
-        bt[0] = bj ? t : bt[0];
-        bs[0] = bj ? s : bs[0];
+        kb[0] = bj ? k : kb[0];
     };
     auto leaf = [&](bool hit, int slot) {
         if (hit) count(S, C_LEAF_TESTS);
@@ -778,9 +775,9 @@ __device__ __forceinline__ int wave_collect(const DeviceScene& S, const Ray& r, 
         const bool ok = cand_test(S, slot, r, tmin, tmax, t);
         // only a candidate that beats the lane's K-th entry changes its buffer; skip the
         // insertion when no lane has one
-        const uint32_t s = uint32_t(slot);
-        const bool ins = hit & ok & ((t < bt[K - 1]) | ((t == bt[K - 1]) & (s < bs[K - 1])));
-        if (__ballot(ins) != 0) insert(ins, t, s);
+        const uint64_t k = kb_key(t, uint32_t(slot));
+        const bool ins = hit & ok & (k < kb[K - 1]);
+        if (__ballot(ins) != 0) insert(ins, k);
     };
     if (need) count(S, C_TRAVERSALS);
     const V3 inv = slab_inv(r.d);
@@ -796,7 +793,7 @@ __device__ __forceinline__ int wave_collect(const DeviceScene& S, const Ray& r, 
             if (need) count(S, C_NODES);
             float tn[4];
             int32_t cl[4];
-            const uint32_t m = need ? wide::test4(nd, r.o, inv, tmin, fminf(tmax, bt[K - 1]), tn) : 0u;
+            const uint32_t m = need ? wide::test4(nd, r.o, inv, tmin, fminf(tmax, kb_t(kb[K - 1])), tn) : 0u;
             wide::links4(nd, cl);
 #pragma unroll
             for (int c = 0; c < 4; ++c) cl[c] = __builtin_amdgcn_readfirstlane(cl[c]);
@@ -845,7 +842,7 @@ __device__ __forceinline__ int wave_collect(const DeviceScene& S, const Ray& r, 
             const float4* nd = S.nodes + 4 * link;
             const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2], n3 = nd[3];
             if (need) count(S, C_NODES);
-            const float cut = fminf(tmax, bt[K - 1]);
+            const float cut = fminf(tmax, kb_t(kb[K - 1]));
             float tl = 0.f, tr = 0.f;
             const bool hl = need && slab(n0, n1, r, inv, tmin, cut, tl);
             const bool hr = need && slab(n2, n3, r, inv, tmin, cut, tr);
@@ -882,9 +879,9 @@ __device__ __forceinline__ int wave_collect(const DeviceScene& S, const Ray& r, 
     int cnt = 0;
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-        L.at(j) = __float_as_uint(bt[j]);
-        L.at(K + j) = bs[j];
-        cnt += bs[j] != kInvalid ? 1 : 0;
+        L.at(j) = uint32_t(kb[j] >> 32);
+        L.at(K + j) = kb_s(kb[j]);
+        cnt += kb_s(kb[j]) != kInvalid ? 1 : 0;
     }
     return cnt;
 }
