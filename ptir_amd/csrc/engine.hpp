@@ -108,26 +108,7 @@ struct Walk {
                 bool hl = slab(n0, n1, r, inv, tmin, cut, tl);
                 bool hr = slab(n2, n3, r, inv, tmin, cut, tr);
                 const int cl = __float_as_int(n0.w), cr = __float_as_int(n1.w);
-                if (hl && cl < 0) { push(~cl); hl = false; }
-                if (hr && cr < 0) { push(~cr); hr = false; }
-                if (hl && hr) {
-                    const bool lf = tl <= tr;
-                    if (sp >= SD) {  // cannot happen after the build-time depth check
-                        atomicOr(S.error_flag, 1);
-                        walk = false;
-                    } else {
-                        L.at(sp++) = uint32_t(lf ? cr : cl);
-                        link = lf ? cl : cr;
-                    }
-                } else if (hl) {
-                    link = cl;
-                } else if (hr) {
-                    link = cr;
-                } else if (sp == 0) {
-                    walk = false;
-                } else {
-                    link = int(L.at(--sp));
-                }
+                lane_step2<SD>(S, L, hl, hr, cl, cr, tl <= tr, link, sp, walk, q0, q1, q2, q3);
             }
         }
         while (__ballot(q0 >= 0) != 0) {

@@ -86,6 +86,39 @@ __device__ __forceinline__ float kb_t(uint64_t k) { return __uint_as_float(uint3
 __device__ __forceinline__ uint32_t kb_s(uint64_t k) { return uint32_t(k); }
 constexpr uint64_t kKbEmpty = (uint64_t(0x7f800000u) << 32) | uint64_t(kInvalid);
 
+// The rest of a per-lane step over a binary node once its two child boxes are tested (hl, hr
+// entered; cl, cr the child links; lf: the left child is entered first): the leaves entered
+// join the leaf queue (left, then right, at its front), the farther of two inner children goes
+// on the stack and the walk moves to the nearer, a single inner child is taken, or the walk
+// pops.  Written with selects so that the lanes of a wave do not split over the cases.
+template <int SD>
+__device__ __forceinline__ void lane_step2(const DeviceScene& S, const Lane& L, bool hl, bool hr, int cl, int cr, bool lf,
+                                           int& link, int& sp, bool& walk, int& q0, int& q1, int& q2, int& q3) {
+    const bool ll = hl && cl < 0, lr = hr && cr < 0;    // leaves entered
+    const bool gl = hl && cl >= 0, gr = hr && cr >= 0;  // inner children entered
+    const int n = int(ll) + int(lr);
+    const int first = ll ? ~cl : ~cr;
+    const int n0 = n == 0 ? q0 : (n == 1 ? first : ~cr);
+    const int n1 = n == 0 ? q1 : (n == 1 ? q0 : ~cl);
+    const int n2 = n == 0 ? q2 : (n == 1 ? q1 : q0);
+    const int n3 = n == 0 ? q3 : (n == 1 ? q2 : q1);
+    q0 = n0;
+    q1 = n1;
+    q2 = n2;
+    q3 = n3;
+    const bool both = gl && gr, none = !gl && !gr;
+    if (both && sp >= SD) {  // cannot happen after the build-time depth check
+        atomicOr(S.error_flag, 1);
+        walk = false;
+        return;
+    }
+    if (both) L.at(sp) = uint32_t(lf ? cr : cl);
+    if (none && sp > 0) link = int(L.at(sp - 1));
+    if (both || !none) link = both ? (lf ? cl : cr) : (gl ? cl : cr);
+    if (none && sp == 0) walk = false;
+    sp += both ? 1 : (none && sp > 0 ? -1 : 0);
+}
+
 // Collect the K closest candidates in (tmin, tmax).  Returns the count; the sorted
 // entries are left in LDS: t at words [0,K), leaf slot at words [K,2K).
 template <int K, int SD>
@@ -196,12 +229,6 @@ __device__ __forceinline__ int collect(const DeviceScene& S, const Ray& r, float
         int sp = 0;
         bool walk = true;             // nodes left to visit
         int q0 = -1, q1 = -1, q2 = -1, q3 = -1;  // queued leaf slots, q0 first
-        auto push = [&](int s) {
-            q3 = q2;
-            q2 = q1;
-            q1 = q0;
-            q0 = s;
-        };
         for (;;) {
             for (;;) {
                 if (__ballot(walk && q0 < 0) == 0) break;  // every walking lane has a leaf queued
@@ -214,26 +241,7 @@ __device__ __forceinline__ int collect(const DeviceScene& S, const Ray& r, float
                     bool hl = slab(n0, n1, r, inv, tmin, cut, tl);
                     bool hr = slab(n2, n3, r, inv, tmin, cut, tr);
                     const int cl = __float_as_int(n0.w), cr = __float_as_int(n1.w);
-                    if (hl && cl < 0) { push(~cl); hl = false; }
-                    if (hr && cr < 0) { push(~cr); hr = false; }
-                    if (hl && hr) {
-                        const bool lf = tl <= tr;
-                        if (sp >= SD) {  // cannot happen after the build-time depth check
-                            atomicOr(S.error_flag, 1);
-                            walk = false;
-                        } else {
-                            L.at(sp++) = uint32_t(lf ? cr : cl);
-                            link = lf ? cl : cr;
-                        }
-                    } else if (hl) {
-                        link = cl;
-                    } else if (hr) {
-                        link = cr;
-                    } else if (sp == 0) {
-                        walk = false;
-                    } else {
-                        link = int(L.at(--sp));
-                    }
+                    lane_step2<SD>(S, L, hl, hr, cl, cr, tl <= tr, link, sp, walk, q0, q1, q2, q3);
                 }
             }
             while (__ballot(q0 >= 0) != 0) {
