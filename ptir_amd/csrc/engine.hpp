@@ -52,15 +52,6 @@ struct Walk {
         q0 = s;
     }
 
-    __device__ __forceinline__ void insert(uint64_t k) {
-#pragma unroll
-        for (int j = 0; j < K; ++j) {
-            const bool lt = k < kb[j];
-            const uint64_t kk = lt ? kb[j] : k;
-            kb[j] = lt ? k : kb[j];
-            k = kk;
-        }
-    }
 
     // One node phase and one leaf phase; every active lane of the wave calls together (a lane
     // with no walk stands by).
@@ -111,20 +102,24 @@ struct Walk {
                 lane_step2<SD>(S, L, hl, hr, cl, cr, tl <= tr, link, sp, walk, q0, q1, q2, q3);
             }
         }
+        // the queue fills from its front: q0 < 0 means empty, and the shift leaves it so
         while (__ballot(q0 >= 0) != 0) {
-            if (q0 >= 0) {
-                const int slot = q0;
-                q0 = q1;
-                q1 = q2;
-                q2 = q3;
-                q3 = -1;
+            const int slot = q0;
+            q0 = q1;
+            q1 = q2;
+            q2 = q3;
+            q3 = -1;
+            bool ins = false;
+            uint64_t k = 0;
+            if (slot >= 0) {
                 float t;
                 count(S, C_LEAF_TESTS);
                 if (cand_test(S, slot, r, tmin, tmax, t)) {
-                    const uint64_t k = kb_key(t, uint32_t(slot));
-                    if (k < kb[K - 1]) insert(k);
+                    k = kb_key(t, uint32_t(slot));
+                    ins = k < kb[K - 1];
                 }
             }
+            if (__ballot(ins) != 0) kb_insert<K>(kb, ins, k);
         }
     }
 };

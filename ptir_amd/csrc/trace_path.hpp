@@ -86,6 +86,23 @@ __device__ __forceinline__ float kb_t(uint64_t k) { return __uint_as_float(uint3
 __device__ __forceinline__ uint32_t kb_s(uint64_t k) { return uint32_t(k); }
 constexpr uint64_t kKbEmpty = (uint64_t(0x7f800000u) << 32) | uint64_t(kInvalid);
 
+// Sorted insertion of k into the lanes' K-buffers, for the lanes with ok set (on which
+// k < kb[K-1]); the other lanes keep theirs.  From the end down, every entry above k's place
+// moves up by one and k lands in the gap: one 64-bit compare and two selects per entry, with
+// no branch, run by the whole wave together.  (Stopping once no lane still moves, or skipping
+// entries known to be empty, costs more in tests and branches than it saves.)
+template <int K>
+__device__ __forceinline__ void kb_insert(uint64_t (&kb)[K], bool ok, uint64_t k) {
+    bool m = ok;  // k's place is at or below j
+#pragma unroll
+    for (int j = K - 1; j >= 1; --j) {
+        const bool bp = m && k < kb[j - 1];
+        kb[j] = bp ? kb[j - 1] : (m ? k : kb[j]);
+        m = bp;
+    }
+    kb[0] = m ? k : kb[0];
+}
+
 // The rest of a per-lane step over a binary node once its two child boxes are tested (hl, hr
 // entered; cl, cr the child links; lf: the left child is entered first): the leaves entered
 // join the leaf queue (left, then right, at its front), the farther of two inner children goes
@@ -126,28 +143,26 @@ __device__ __forceinline__ int collect(const DeviceScene& S, const Ray& r, float
     uint64_t kb[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) kb[j] = kKbEmpty;
-    unsigned long long ncand = 0;
-    auto insert = [&](uint64_t k) {
-#pragma unroll
-        for (int j = 0; j < K; ++j) {
-            const bool lt = k < kb[j];
-            const uint64_t kk = lt ? kb[j] : k;
-            kb[j] = lt ? k : kb[j];
-            k = kk;
+    // The leaf tests of the lanes that have one (has), then the wave's insertion outside the
+    // branches on has and on the test (measured: the forward is 10% faster so than with the
+    // insertion and the queue shift inside them).
+    auto leaf = [&](bool has, int slot) {
+        bool ins = false;
+        uint64_t k = 0;
+        if (has) {
+            float t;
+            count(S, C_LEAF_TESTS);
+            if (cand_test(S, slot, r, tmin, tmax, t)) {
+                k = kb_key(t, uint32_t(slot));
+                ins = k < kb[K - 1];
+            }
         }
-    };
-    auto leaf = [&](int slot) {
-        float t;
-        count(S, C_LEAF_TESTS);
-        if (cand_test(S, slot, r, tmin, tmax, t)) {
-            const uint64_t k = kb_key(t, uint32_t(slot));
-            if (k < kb[K - 1]) insert(k);
-        }
+        if (__ballot(ins) != 0) kb_insert<K>(kb, ins, k);
     };
     count(S, C_TRAVERSALS);
     const V3 inv = slab_inv(r.d);
     if (S.n > 0 && S.root < 0) {
-        leaf(~S.root);
+        leaf(true, ~S.root);
     } else if (S.n > 0 && (S.wide_walks & 2)) {
         // Node steps and leaf tests run in separate phases so that each phase keeps most
         // lanes of the wave busy.  A lane with an empty leaf queue steps through one 4-wide
@@ -206,19 +221,19 @@ __device__ __forceinline__ int collect(const DeviceScene& S, const Ray& r, float
                     }
                 }
             }
+            // a queue is filled from its front, so q0 < 0 means it is empty and the shift
+            // leaves it so
             while (__ballot(q0 >= 0) != 0) {
-                if (q0 >= 0) {
-                    const int slot = q0;
-                    q0 = q1;
-                    q1 = q2;
-                    q2 = q3;
-                    q3 = q4;
-                    q4 = q5;
-                    q5 = q6;
-                    q6 = q7;
-                    q7 = -1;
-                    leaf(slot);
-                }
+                const int slot = q0;
+                q0 = q1;
+                q1 = q2;
+                q2 = q3;
+                q3 = q4;
+                q4 = q5;
+                q5 = q6;
+                q6 = q7;
+                q7 = -1;
+                leaf(slot >= 0, slot);
             }
             if (__ballot(walk) == 0) break;
         }
@@ -245,19 +260,16 @@ __device__ __forceinline__ int collect(const DeviceScene& S, const Ray& r, float
                 }
             }
             while (__ballot(q0 >= 0) != 0) {
-                if (q0 >= 0) {
-                    const int slot = q0;
-                    q0 = q1;
-                    q1 = q2;
-                    q2 = q3;
-                    q3 = -1;
-                    leaf(slot);
-                }
+                const int slot = q0;
+                q0 = q1;
+                q1 = q2;
+                q2 = q3;
+                q3 = -1;
+                leaf(slot >= 0, slot);
             }
             if (__ballot(walk) == 0) break;
         }
     }
-    (void)ncand;
     int cnt = 0;
 #pragma unroll
     for (int j = 0; j < K; ++j) {
@@ -765,18 +777,6 @@ __device__ __forceinline__ int wave_collect(const DeviceScene& S, const Ray& r, 
     uint64_t kb[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) kb[j] = kKbEmpty;
-    // branch-free sorted insertion (all lanes run it together); ok = false leaves it unchanged
-    auto insert = [&](bool ok, uint64_t k) {
-        auto below = [&](int j) { return ok & (k < kb[j]); };
-        bool bj = below(K - 1);
-#pragma unroll
-        for (int j = K - 1; j >= 1; --j) {
-            const bool bp = below(j - 1);
-            kb[j] = bp ? kb[j - 1] : (bj ? k : kb[j]);
-            bj = bp;
-        }
-        kb[0] = bj ? k : kb[0];
-    };
     auto leaf = [&](bool hit, int slot) {
         if (hit) count(S, C_LEAF_TESTS);
         float t;
@@ -785,7 +785,7 @@ __device__ __forceinline__ int wave_collect(const DeviceScene& S, const Ray& r, 
         // insertion when no lane has one
         const uint64_t k = kb_key(t, uint32_t(slot));
         const bool ins = hit & ok & (k < kb[K - 1]);
-        if (__ballot(ins) != 0) insert(ins, k);
+        if (__ballot(ins) != 0) kb_insert<K>(kb, ins, k);
     };
     if (need) count(S, C_TRAVERSALS);
     const V3 inv = slab_inv(r.d);
