@@ -200,7 +200,13 @@ __device__ __forceinline__ void hit_bwd(const DeviceScene& S, const BwdIO& io, c
     const V3 dRGB = v3(raw.x > 0.f ? b.radG.x * weight : 0.f, raw.y > 0.f ? b.radG.y * weight : 0.f,
                        raw.z > 0.f ? b.radG.z * weight : 0.f);
     float* dsh = io.d_sh + size_t(S.sh_stride) * id;
-    for (int k = 0; k < ncoef; ++k) {
+    // The lanes of a coherent wave often hold the same Gaussian at the same step: each lane
+    // starts its coefficients at its own offset, so one atomic instruction of the wave
+    // spreads over different addresses (measured: backward 85.5 -> 75.2 ms on the
+    // object-scale rays).
+    const int off = int(threadIdx.x & 63u) % ncoef;
+    for (int kk = 0; kk < ncoef; ++kk) {
+        const int k = kk + off < ncoef ? kk + off : kk + off - ncoef;
         const float bk = sh_basis(k, ray.d);
         atomicAdd(dsh + 3 * k, bk * dRGB.x);
         atomicAdd(dsh + 3 * k + 1, bk * dRGB.y);
