@@ -175,16 +175,6 @@ __device__ __forceinline__ int collect(const DeviceScene& S, const Ray& r, float
         int sp = 0;
         bool walk = true;             // nodes left to visit
         int q0 = -1, q1 = -1, q2 = -1, q3 = -1, q4 = -1, q5 = -1, q6 = -1, q7 = -1;  // queued leaf slots
-        auto push = [&](int s) {
-            q7 = q6;
-            q6 = q5;
-            q5 = q4;
-            q4 = q3;
-            q3 = q2;
-            q2 = q1;
-            q1 = q0;
-            q0 = s;
-        };
         for (;;) {
             for (;;) {
                 if (__ballot(walk && q0 < 0) == 0) break;  // every walking lane has a leaf queued
@@ -195,30 +185,39 @@ __device__ __forceinline__ int collect(const DeviceScene& S, const Ray& r, float
                     int32_t cl[4];
                     const uint32_t m = wide::test4(nd, r.o, inv, tmin, fminf(tmax, kb_t(kb[K - 1])), tn);
                     wide::links4(nd, cl);
+                    // leaves entered go to the queue's front (child order), then the nearest
+                    // inner child is taken and the others pushed (farthest deepest), or the
+                    // walk pops: written with selects, so the lanes do not split over the cases
                     int ni = 0;
 #pragma unroll
                     for (int c = 0; c < 4; ++c) {
                         const bool hit = ((m >> c) & 1u) != 0u && cl[c] != wide::kEmpty;
-                        if (hit && cl[c] < 0) push(~cl[c]);
+                        const bool lf = hit && cl[c] < 0;
+                        q7 = lf ? q6 : q7;
+                        q6 = lf ? q5 : q6;
+                        q5 = lf ? q4 : q5;
+                        q4 = lf ? q3 : q4;
+                        q3 = lf ? q2 : q3;
+                        q2 = lf ? q1 : q2;
+                        q1 = lf ? q0 : q1;
+                        q0 = lf ? ~cl[c] : q0;
                         const bool in = hit && cl[c] >= 0;
                         ni += in ? 1 : 0;
                         tn[c] = in ? tn[c] : __int_as_float(0x7f800000);
                     }
-                    if (ni == 0) {
-                        if (sp == 0) walk = false;
-                        else link = int(L.at(--sp));
-                    } else {
-                        wide::sort4(tn, cl);
-                        if (sp + ni - 1 > SD) {  // cannot happen after the build-time stack check
-                            atomicOr(S.error_flag, 1);
-                            walk = false;
-                        } else {
-#pragma unroll
-                            for (int j = 3; j >= 1; --j)
-                                if (j < ni) L.at(sp++) = uint32_t(cl[j]);
-                            link = cl[0];
-                        }
+                    wide::sort4(tn, cl);
+                    const bool pop = ni == 0;
+                    const bool over = !pop && sp + ni - 1 > SD;  // cannot happen after the build-time stack check
+                    if (over) atomicOr(S.error_flag, 1);
+                    const int top = int(L.at(sp > 0 ? sp - 1 : 0));
+                    if (!pop && !over) {
+                        if (ni > 3) L.at(sp + ni - 4) = uint32_t(cl[3]);
+                        if (ni > 2) L.at(sp + ni - 3) = uint32_t(cl[2]);
+                        if (ni > 1) L.at(sp + ni - 2) = uint32_t(cl[1]);
                     }
+                    walk = walk && !over && !(pop && sp == 0);
+                    link = pop ? (sp > 0 ? top : link) : cl[0];
+                    sp = pop ? (sp > 0 ? sp - 1 : 0) : (over ? sp : sp + ni - 1);
                 }
             }
             // a queue is filled from its front, so q0 < 0 means it is empty and the shift
