@@ -21,6 +21,7 @@ struct ptir_tracer {
     int32_t wide_nodes = 0;    // 4-wide nodes of the built hierarchy
     void* wf = nullptr;        // wavefront-schedule workspace
     uint64_t wf_bytes = 0;
+    ptir::GrtRecBuffers grt_rec;  // hit records of the 3DGRT backward
     std::string err;
 };
 
@@ -230,6 +231,7 @@ ptir_status ptir_destroy(ptir_tracer* t) {
     if (!t) return PTIR_OK;
     ptir::bvh_free(t->bvh);
     if (t->wf) (void)hipFree(t->wf);
+    ptir::grt_rec_free(t->grt_rec);
     delete t;
     return PTIR_OK;
 }
@@ -564,7 +566,15 @@ ptir_status ptir_grt_trace_bwd(ptir_tracer* t, const ptir_grt_scene* g, const pt
                               in->hit_distance2_grad, in->distortion_grad, in->shading_normal_grad,
                               g->particle_density, g->shading_normal, out->density_grad, out->radiance_grad,
                               g->shading_normal ? out->shading_normal_grad : nullptr};
-    e = ptir::launch_grt_bwd(S, io, t->opts.kbuffer, t->short_stack, s);
+    // Hit records (64 B per slot, 32 slots per pixel) replace most of the per-hit gradient
+    // atomics; where they cannot be had (slot indices past 32 bits, or no memory for them),
+    // every hit adds its gradients by atomics.
+    const ptir::GrtRecBuffers* rec = nullptr;
+    if (paths > 0 && N > 0 && uint64_t(paths) * ptir::grt::kRecPerPixel < (uint64_t(1) << 32)) {
+        if (ptir::grt_rec_reserve(t->grt_rec, size_t(paths), N) == hipSuccess) rec = &t->grt_rec;
+        else (void)hipGetLastError();
+    }
+    e = ptir::launch_grt_bwd(S, io, t->opts.kbuffer, t->short_stack, rec, N, s);
     if (e != hipSuccess) return hip_fail(t, e, "primary-only backward launch");
     return PTIR_OK;
 }

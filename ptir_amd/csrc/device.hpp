@@ -219,7 +219,16 @@ struct BwdIO {
     const float* density;  // [N,12] position, density, quaternion (w,x,y,z), scale, pad
     const float* snormal;  // [N,3] or null
     float *d_density, *d_sh, *d_snormal;  // [N,12], [N, sh_stride], [N,3] (zeroed by the caller)
+    // Hit records (null: every hit adds its gradients by atomics).  Record j of pixel pix is
+    // slot j * paths + pix: rec[4 * slot .. 4 * slot + 3] and its Gaussian rec_key[slot];
+    // rec_cnt[pix] counts the pixel's records (at most kRecPerPixel).
+    float4* rec;
+    uint32_t* rec_key;
+    uint32_t* rec_cnt;
+    uint32_t paths;
 };
+// Records per pixel of the 3DGRT backward; hits beyond them add their gradients by atomics.
+constexpr uint32_t kRecPerPixel = 32;
 }  // namespace grt
 
 enum CounterId { C_PATHS = 0, C_SEGMENTS, C_CANDIDATES, C_ACCEPTED, C_ACCEPTED_PRIMARY, C_SHADOW_CANDIDATES,

@@ -151,11 +151,15 @@ struct BwdPixel {
     V3 rad, sn;
 };
 
-// gaussianParticles.cuh:473-772 processHitBwd for one candidate.
-__device__ __forceinline__ void hit_bwd(const DeviceScene& S, const BwdIO& io, const Ray& ray, int slot, BwdPixel& b) {
+// gaussianParticles.cuh:473-772 processHitBwd for one candidate.  Returns whether the hit
+// counts (alpha > 0).  With a record R (4 float4) and its key Rk, the hit's gradients are not
+// added here: R holds the per-hit values they derive from (dRGB and weight; the position,
+// density, quaternion and scale gradients), Rk the Gaussian, and k_grt_sum folds them.
+__device__ __forceinline__ bool hit_bwd(const DeviceScene& S, const BwdIO& io, const Ray& ray, int slot, BwdPixel& b,
+                                        float4* R, uint32_t* Rk) {
     HitGeom h;
     const float galpha = hit_alpha(S, slot, ray, h);
-    if (!(galpha > 0.f)) return;
+    if (!(galpha > 0.f)) return false;
     const uint32_t id = orig_id(S, slot);
     const float* pd = io.density + 12 * size_t(id);
     const V3 s = v3(pd[8], pd[9], pd[10]);
@@ -205,7 +209,7 @@ __device__ __forceinline__ void hit_bwd(const DeviceScene& S, const BwdIO& io, c
     // spreads over different addresses (measured: backward 85.5 -> 75.2 ms on the
     // object-scale rays).
     const int off = int(threadIdx.x & 63u) % ncoef;
-    for (int kk = 0; kk < ncoef; ++kk) {
+    for (int kk = 0; kk < (R ? 0 : ncoef); ++kk) {
         const int k = kk + off < ncoef ? kk + off : kk + off - ncoef;
         const float bk = sh_basis(k, ray.d);
         atomicAdd(dsh + 3 * k, bk * dRGB.x);
@@ -221,7 +225,7 @@ __device__ __forceinline__ void hit_bwd(const DeviceScene& S, const BwdIO& io, c
     b.sn += psn * weight;
     const V3 residSN = last ? v3s(0.f) : (b.snInt - b.sn) * (1.f / nextT);
     const float aSnG = b.trm * dot(psn - residSN, b.snG);
-    if (io.d_snormal) {
+    if (io.d_snormal && !R) {
         float* dn = io.d_snormal + 3 * size_t(id);
         atomicAdd(dn, weight * b.snG.x);
         atomicAdd(dn + 1, weight * b.snG.y);
@@ -229,7 +233,7 @@ __device__ __forceinline__ void hit_bwd(const DeviceScene& S, const BwdIO& io, c
     }
     const float common = aHitG + aDnsG + aSnG + b.trm * dot(col - residRc, b.radG);
     float* gd = io.d_density + 12 * size_t(id);
-    atomicAdd(gd + 3, gres * common);
+    if (!R) atomicAdd(gd + 3, gres * common);
     const float grayDistG = response_grad(S.kernel_degree, h.gd, gres, sigma * common);
     const V3 gcrod = cross(grd, gro);
     const V3 gcrodG = gcrod * (2.f * grayDistG);
@@ -249,9 +253,11 @@ __device__ __forceinline__ void hit_bwd(const DeviceScene& S, const BwdIO& io, c
     const V3 gposcG = v3(dot(r0, gposcrG), dot(r1, gposcrG), dot(r2, gposcrG));
     float dq[4] = {0.f, 0.f, 0.f, 0.f};
     quat_grad(gposc, gposcrG, q, dq);
-    atomicAdd(gd + 0, -gposcG.x);
-    atomicAdd(gd + 1, -gposcG.y);
-    atomicAdd(gd + 2, -gposcG.z);
+    if (!R) {
+        atomicAdd(gd + 0, -gposcG.x);
+        atomicAdd(gd + 1, -gposcG.y);
+        atomicAdd(gd + 2, -gposcG.z);
+    }
     // grd = safe_normalize(grdu)
     const V3 gin = grdG + grdHitG;
     const float l2 = dot(grdu, grdu);
@@ -261,15 +267,38 @@ __device__ __forceinline__ void hit_bwd(const DeviceScene& S, const BwdIO& io, c
         grduG = gin * il - grdu * (dot(gin, grdu) * il * il * il);
     }
     const V3 sclDir = v3(-rayDirR.x * is.x * is.x, -rayDirR.y * is.y * is.y, -rayDirR.z * is.z * is.z) * grduG;
-    atomicAdd(gd + 8, sclHitG.x + sclGro.x + sclDir.x);
-    atomicAdd(gd + 9, sclHitG.y + sclGro.y + sclDir.y);
-    atomicAdd(gd + 10, sclHitG.z + sclGro.z + sclDir.z);
+    const V3 gscl = v3(sclHitG.x + sclGro.x + sclDir.x, sclHitG.y + sclGro.y + sclDir.y, sclHitG.z + sclGro.z + sclDir.z);
     quat_grad(ray.d, is * grduG, q, dq);
-    atomicAdd(gd + 4, dq[0]);
-    atomicAdd(gd + 5, dq[1]);
-    atomicAdd(gd + 6, dq[2]);
-    atomicAdd(gd + 7, dq[3]);
+    if (R) {
+        R[0] = make_float4(dRGB.x, dRGB.y, dRGB.z, weight);
+        R[1] = make_float4(-gposcG.x, -gposcG.y, -gposcG.z, gres * common);
+        R[2] = make_float4(dq[0], dq[1], dq[2], dq[3]);
+        R[3] = make_float4(gscl.x, gscl.y, gscl.z, 0.f);
+        *Rk = id;
+    } else {
+        atomicAdd(gd + 8, gscl.x);
+        atomicAdd(gd + 9, gscl.y);
+        atomicAdd(gd + 10, gscl.z);
+        atomicAdd(gd + 4, dq[0]);
+        atomicAdd(gd + 5, dq[1]);
+        atomicAdd(gd + 6, dq[2]);
+        atomicAdd(gd + 7, dq[3]);
+    }
     b.trm = nextT;
+    return true;
+}
+
+// The hits of pixel pix go to its records while they last (nrec counts them), then to atomics.
+__device__ __forceinline__ void hit_bwd_pix(const DeviceScene& S, const BwdIO& io, const Ray& ray, int slot, BwdPixel& b,
+                                            size_t pix, uint32_t& nrec) {
+    float4* R = nullptr;
+    uint32_t* Rk = nullptr;
+    if (io.rec && nrec < kRecPerPixel) {
+        const size_t s = size_t(nrec) * io.paths + pix;
+        R = io.rec + 4 * s;
+        Rk = io.rec_key + s;
+    }
+    if (hit_bwd(S, io, ray, slot, b, R, Rk) && R) ++nrec;
 }
 
 // threedgrt_tracer/src/kernels/cuda/referenceBwdOptix.cu:103-184 for one pixel: the forward's
@@ -302,6 +331,7 @@ __device__ __forceinline__ void bwd_path(const DeviceScene& S, const BwdIO& io, 
     float startT = fmaxf(0.f, t0 - kEpsT);
     const float endT = fminf(maxHit, t1) + kEpsT;
     count(S, C_PATHS);
+    uint32_t nrec = 0;
     if (wstack) {
         bool done = false;
         for (;;) {
@@ -312,22 +342,89 @@ __device__ __forceinline__ void bwd_path(const DeviceScene& S, const BwdIO& io, 
                 if (cnt == 0) done = true;
                 for (int k = 0; k < cnt; ++k) {
                     const float t = __uint_as_float(L.at(k));
-                    hit_bwd(S, io, ray, int(L.at(K + k)), b);
+                    hit_bwd_pix(S, io, ray, int(L.at(K + k)), b, pix, nrec);
                     startT = fmaxf(startT, t);
                 }
             }
         }
-        return;
-    }
-    while (startT < endT) {
-        const int cnt = collect<K, SD>(S, ray, startT + kEpsT, endT, L);
-        if (cnt == 0) break;
-        for (int k = 0; k < cnt; ++k) {
-            const float t = __uint_as_float(L.at(k));
-            hit_bwd(S, io, ray, int(L.at(K + k)), b);
-            startT = fmaxf(startT, t);
+    } else {
+        while (startT < endT) {
+            const int cnt = collect<K, SD>(S, ray, startT + kEpsT, endT, L);
+            if (cnt == 0) break;
+            for (int k = 0; k < cnt; ++k) {
+                const float t = __uint_as_float(L.at(k));
+                hit_bwd_pix(S, io, ray, int(L.at(K + k)), b, pix, nrec);
+                startT = fmaxf(startT, t);
+            }
         }
     }
+    if (io.rec) io.rec_cnt[pix] = nrec;
+}
+
+// k_grt_sum for Gaussian id: the gradients of its n records (their slots listed at
+// bucket[0, n)), added to what the hits beyond the records left by atomics.  One lane per
+// Gaussian, so no other lane writes its gradients.
+__device__ __forceinline__ void sum_records(const DeviceScene& S, const BwdIO& io, uint32_t id, const uint32_t* bucket,
+                                            uint32_t n) {
+    const int ncoef = (S.sph_degree + 1) * (S.sph_degree + 1);
+    float sh[48];
+#pragma unroll
+    for (int i = 0; i < 48; ++i) sh[i] = 0.f;
+    V3 dn = v3s(0.f), gpos = v3s(0.f), gscl = v3s(0.f);
+    float gdens = 0.f, dq[4] = {0.f, 0.f, 0.f, 0.f};
+    for (uint32_t i = 0; i < n; ++i) {
+        const size_t s = bucket[i];
+        const size_t pix = s % io.paths;
+        const float4* R = io.rec + 4 * s;
+        const float4 r0 = R[0], r1 = R[1], r2 = R[2], r3 = R[3];
+        const V3 d = primary_ray(S, pix).d;
+        const V3 dRGB = v3(r0.x, r0.y, r0.z);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            if (k < ncoef) {
+                const float bk = sh_basis(k, d);
+                sh[3 * k] += bk * dRGB.x;
+                sh[3 * k + 1] += bk * dRGB.y;
+                sh[3 * k + 2] += bk * dRGB.z;
+            }
+        }
+        if (io.snrm_grad) dn += v3(io.snrm_grad[3 * pix], io.snrm_grad[3 * pix + 1], io.snrm_grad[3 * pix + 2]) * r0.w;
+        gpos += v3(r1.x, r1.y, r1.z);
+        gdens += r1.w;
+        dq[0] += r2.x;
+        dq[1] += r2.y;
+        dq[2] += r2.z;
+        dq[3] += r2.w;
+        gscl += v3(r3.x, r3.y, r3.z);
+    }
+    if (n == 0) return;
+    float* dsh = io.d_sh + size_t(S.sh_stride) * id;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        if (k < ncoef) {
+            dsh[3 * k] += sh[3 * k];
+            dsh[3 * k + 1] += sh[3 * k + 1];
+            dsh[3 * k + 2] += sh[3 * k + 2];
+        }
+    }
+    if (io.d_snormal) {
+        float* o = io.d_snormal + 3 * size_t(id);
+        o[0] += dn.x;
+        o[1] += dn.y;
+        o[2] += dn.z;
+    }
+    float* gd = io.d_density + 12 * size_t(id);
+    gd[0] += gpos.x;
+    gd[1] += gpos.y;
+    gd[2] += gpos.z;
+    gd[3] += gdens;
+    gd[4] += dq[0];
+    gd[5] += dq[1];
+    gd[6] += dq[2];
+    gd[7] += dq[3];
+    gd[8] += gscl.x;
+    gd[9] += gscl.y;
+    gd[10] += gscl.z;
 }
 
 }  // namespace grt

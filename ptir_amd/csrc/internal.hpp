@@ -113,7 +113,27 @@ hipError_t resolve_backward(const ResolveIO& io, const ResolveGrads& g, hipStrea
 hipError_t launch_trace_fwd(const DeviceScene& S, const FwdOut& out, int kbuf, bool short_stack, hipStream_t s);
 // 3DGRT primary-only tracer (grt.hip)
 hipError_t launch_grt_fwd(const DeviceScene& S, const grt::FwdOut& out, int kbuf, bool short_stack, hipStream_t s);
-hipError_t launch_grt_bwd(const DeviceScene& S, const grt::BwdIO& io, int kbuf, bool short_stack, hipStream_t s);
+// Hit records of the 3DGRT backward and the per-Gaussian lists that fold them (grt.hip): the
+// tracer keeps them between calls and grows them with the launch (records: 64 B per slot,
+// kRecPerPixel slots per pixel).
+struct GrtRecBuffers {
+    float4* rec = nullptr;        // [paths * kRecPerPixel * 4]
+    uint32_t* key = nullptr;      // [paths * kRecPerPixel] Gaussian of each slot
+    uint32_t* cnt = nullptr;      // [paths] records per pixel
+    uint32_t* bucket = nullptr;   // [paths * kRecPerPixel] slots grouped by Gaussian
+    uint32_t* gcount = nullptr;   // [N] records per Gaussian
+    uint32_t* gcursor = nullptr;  // [N]
+    uint32_t* goff = nullptr;     // [N] exclusive prefix sum of gcount
+    void* scan_tmp = nullptr;
+    size_t scan_bytes = 0;
+    size_t paths_cap = 0, n_cap = 0;
+};
+hipError_t grt_rec_reserve(GrtRecBuffers& r, size_t paths, size_t n);
+void grt_rec_free(GrtRecBuffers& r);
+// rec: the record buffers reserved for this launch (then io's record fields are set from
+// them), or null (every hit adds its gradients by atomics).
+hipError_t launch_grt_bwd(const DeviceScene& S, const grt::BwdIO& io, int kbuf, bool short_stack, const GrtRecBuffers* rec,
+                          size_t n, hipStream_t s);
 hipError_t launch_trace_bwd(const DeviceScene& S, const BwdIO& io, int kbuf, bool short_stack, hipStream_t s);
 // Wavefront schedule of the forward (wavefront.hip); `work` holds wavefront_bytes(paths, max_bounces).
 uint64_t wavefront_bytes(uint64_t paths, uint32_t max_bounces);

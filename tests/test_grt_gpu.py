@@ -36,9 +36,14 @@ def _gpu(tr, sc, device):
     return tr.trace(*args, 0, sc.sph_degree, sc.min_transmittance), args
 
 
-@pytest.mark.parametrize("seed", [3, 9])
-def test_grt_forward_and_backward_match_the_oracle(device, seed):
-    sc = _scene(seed)
+@pytest.mark.parametrize("seed,dense", [(3, False), (9, False), (5, True)])
+def test_grt_forward_and_backward_match_the_oracle(device, seed, dense):
+    """dense: four times the Gaussians at 0.3 times the density, so that rays cross many of
+    them before the transmittance ends: a third of the pixels then hold more hits than the
+    backward's 32 records per pixel, and those past the records add their gradients by atomics."""
+    sc = _scene(seed, n=8192 if dense else 2048)
+    if dense:
+        sc.density = sc.density * 0.3
     inp = O.grt_inputs(sc.particle_density(), sc.features, sc.shading_normal, sph_degree=sc.sph_degree,
                        min_transmittance=sc.min_transmittance)
     rays = O.Rays(sc.rays_ori, sc.rays_dir, sc.ray_to_world)
@@ -55,6 +60,10 @@ def test_grt_forward_and_backward_match_the_oracle(device, seed):
     vis = outs[8].view(torch.int32).cpu().numpy().reshape(-1)
     assert np.array_equal(vis != 0, ref["visibility"] > 0)
     tr.check()
+    most = float(outs[7].max())
+    print(f"grt seed {seed}: at most {most:.0f} hits per pixel", flush=True)
+    if dense:
+        assert most > 40, most  # past the 32 records per pixel
 
     rng = np.random.default_rng(seed)
     B, H, W = sc.rays_dir.shape[:3]
