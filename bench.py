@@ -20,10 +20,16 @@ a strided sample of the same paths on this host's cores.
 from __future__ import annotations
 
 import argparse
+import csv
+import glob
 import json
 import os
 import re
+import shutil
+import signal
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -31,7 +37,9 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 
-# Algorithmic bytes per work unit (see DESIGN.md "Measurement").
+# Algorithmic bytes per work unit: what each event the kernels count moves (a candidate's
+# geometry record, an accepted hit's material and normal, a primary hit's SH coefficients, an
+# env texel fetch, a gradient read-modify-write, a path's inputs and outputs, tape records).
 B_CAND, B_ACC, B_SH, B_ENV, B_MAT_ATOM, B_ENV_ATOM, B_IO_FWD, B_IO_BWD = 44, 32, 192, 64, 40, 96, 152, 92
 B_TAPE_PATH, B_TAPE_SEG, B_TAPE_HIT = 64, 64, 8  # path tape: record header, segment record, hit entry
 
@@ -65,6 +73,68 @@ def read_traffic(kernel: str):
         if m and m.group(1) == kernel:
             return float(m.group(2))
     return None
+
+
+def _pmc_values(outdir: str) -> dict:
+    """{kernel name: {dispatch id: counter value summed over its rows}} of one rocprofv3 --pmc run."""
+    vals: dict = {}
+    for path in glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True):
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                d = vals.setdefault(r["Kernel_Name"], {})
+                d[int(r["Dispatch_Id"])] = d.get(int(r["Dispatch_Id"]), 0.0) + float(r["Counter_Value"])
+    return vals
+
+
+def _pmc_run(counter: str, argv: list, timeout: int) -> dict:
+    """One `rocprofv3 --pmc <counter>` child run of `argv` (from /tmp, TMPDIR=/tmp)."""
+    outdir = tempfile.mkdtemp(prefix="ptir_pmc_", dir="/tmp")
+    cmd = ["rocprofv3", "--pmc", counter, "-d", outdir, "-o", "pmc", "--output-format", "csv", "--"] + argv
+    env = dict(os.environ, TMPDIR="/tmp")
+    p = subprocess.Popen(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                         start_new_session=True)
+    try:
+        p.wait(timeout=timeout)
+    except subprocess.TimeoutExpired:
+        os.killpg(p.pid, signal.SIGKILL)
+        p.wait()
+        shutil.rmtree(outdir, ignore_errors=True)
+        raise RuntimeError(f"rocprofv3 --pmc {counter} did not finish in {timeout} s")
+    # rocprofv3 itself may end with a signal after the program and its results are complete
+    vals = _pmc_values(outdir)
+    shutil.rmtree(outdir, ignore_errors=True)
+    if not vals:
+        raise RuntimeError(f"rocprofv3 --pmc {counter} wrote no counters (exit {p.returncode})")
+    return vals
+
+
+def measure_traffic(tcfg: dict, tape: bool) -> dict:
+    """HBM bytes per launch of the tracer kernels, measured on this build: FETCH_SIZE and
+    WRITE_SIZE (KiB per dispatch) of one object-scale step in child runs under rocprofv3, each
+    scaled by what the same counter reads for a 1 GiB device copy.  A kernel's figure is its
+    timed dispatch (the last; the first is the counting step), summed over its instances."""
+    me = [sys.executable, os.path.join(ROOT, "bench.py")]
+    probe = me + ["--no-cpu-baseline", "--no-nrc", "--no-workloads", "--no-traffic", "--steps", "1", "--warmup", "0",
+                  "--tracer-config", json.dumps(tcfg)] + ([] if tape else ["--no-tape"])
+    copy = me + ["--copy-calibration", str(1 << 30)]
+    got = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        print(f"traffic: rocprofv3 --pmc {counter}", file=sys.stderr, flush=True)
+        step = _pmc_run(counter, probe, 300)
+        cal = _pmc_run(counter, copy, 120)
+        copies = [v for k, v in cal.items() if "copyBuffer" in k or "copy" in k.lower()]
+        ref = max((max(d.values()) for d in copies), default=0.0)
+        if ref <= 0:
+            raise RuntimeError(f"no copy kernel in the {counter} calibration run")
+        scale = (1 << 30) / 1024 / ref  # bytes per KiB counted
+        for name, d in step.items():
+            last = d[max(d)] * 1024 * scale
+            for short in ("k_trace_fwd", "k_trace_bwd", "k_wf_primary", "k_wf_nee", "k_wf_bounce"):
+                if short in name:
+                    got.setdefault(short, {}).setdefault(counter, 0.0)
+                    got[short][counter] += last
+        got.setdefault("_scale", {})[counter] = round(scale, 3)
+    return got
 
 
 def cpu_baseline(scene, table, stride: int) -> dict:
@@ -310,6 +380,11 @@ def main() -> int:
                     help="skip the batched (ragged, 500k) and scene-scale (3M) inversion-step timings")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only copy a float32 buffer of BYTES bytes 5 times (HBM counter calibration)")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="take roofline.traffic from profiles/traffic_object.md instead of measuring it "
+                         "with rocprofv3 --pmc child runs of this build")
+    ap.add_argument("--tracer-config", default="", metavar="JSON",
+                    help="tracer configuration to use instead of tuning (the traffic runs pass the tuned one)")
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
@@ -350,7 +425,9 @@ def main() -> int:
               max_bounces=scene.max_bounces)
     tuned = None
     tcfg = dict(kbuffer=16, traversal="wave", schedule="megakernel", builder="ploc", wide_walks="all")
-    if not args.no_autotune:
+    if args.tracer_config:
+        tcfg = json.loads(args.tracer_config)
+    elif not args.no_autotune:
         print("autotuning the object-scale step", file=sys.stderr, flush=True)
         tuned = autotune((g.positions, g.rotation, g.scale, g.density), fargs[4:], scene.frame, g.ray_to_world,
                          g.rays_ori, g.rays_dir, tracer_kwargs=dict(enable_metallic=True), trace_kwargs=kw,
@@ -401,8 +478,27 @@ def main() -> int:
     fwd_name = "k_trace_fwd" if tcfg["schedule"] == "megakernel" else "k_wf_primary+k_wf_nee+k_wf_bounce"
     dom, ms_dom, by = ("k_trace_bwd", ms_bwd, bb) if ms_bwd >= ms_fwd else (fwd_name, ms_fwd, fb)
     achieved = by / (ms_dom * 1e-3) / 1e9
+    traffic, source = None, ""
+    if not args.no_traffic:
+        try:
+            meas = measure_traffic(tcfg, tape is not None)
+            parts = [meas.get(k) for k in dom.split("+")]
+            if all(p and "FETCH_SIZE" in p and "WRITE_SIZE" in p for p in parts):
+                traffic = sum(p["FETCH_SIZE"] + p["WRITE_SIZE"] for p in parts)
+                sc = meas["_scale"]
+                source = (f"measured on this build: rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, child runs of one "
+                          f"step, the timed dispatch, scaled by a 1 GiB copy (x{sc['FETCH_SIZE']} / "
+                          f"x{sc['WRITE_SIZE']})")
+            else:
+                source = "measurement found no dispatch of the kernel"
+        except (OSError, RuntimeError, ValueError, KeyError) as exc:
+            source = f"measurement failed ({exc})"
+    if traffic is None:
+        traffic = read_traffic(dom)
+        source = ("profiles/traffic_object.md (committed rocprofv3 PMC summary)" +
+                  (f"; {source}" if source else ""))
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": read_traffic(dom),
+                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic, "traffic_source": source,
                 "kernel": dom, "algorithmic_bytes": by, "ms_per_launch": round(ms_dom, 3),
                 "ms_fwd": round(ms_fwd, 3), "ms_bwd": round(ms_bwd, 3)}
     line = {
