@@ -162,7 +162,9 @@ def cpu_baseline(scene, table, stride: int) -> dict:
 
 def time_primary_only(steps: int = 10, warmup: int = 2) -> dict:
     """The 3DGRT primary-only tracer on the object-scale rays: G-buffer forward and the
-    backward to every Gaussian parameter (random output gradients), timed with HIP events."""
+    backward to every Gaussian parameter (random output gradients), timed with HIP events.
+    The forward records its hits on a tape (64 per pixel) that the backward reads, as the
+    autograd Tracer does; `ms_bwd_retrace` is the backward that re-traces every pixel."""
     import torch
 
     from ptir_amd import scenes
@@ -177,26 +179,33 @@ def time_primary_only(steps: int = 10, warmup: int = 2) -> dict:
         args = (0, g.ray_to_world, g.rays_ori, g.rays_dir, g.particle_density(), g.features, g.shading_normal)
         gen = torch.Generator(device=dev).manual_seed(0)
         tr.build_bvh(g.positions, g.rotation, g.scale, g.density)
+        tape = tr.new_tape(g.rays_dir, hits_per_pixel=64)
         outs = tr.trace(*args, 0, sc.sph_degree, sc.min_transmittance)
         grads = [torch.randn(o.shape, device=dev, generator=gen) for o in outs[:7]]
         grads[2] = grads[2][..., :1].contiguous()
-        ev = []
+        ev, ev_re = [], []
         for i in range(warmup + steps):
-            e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+            e0, e1, e2, e3 = (torch.cuda.Event(enable_timing=True) for _ in range(4))
             tr.build_bvh(g.positions, g.rotation, g.scale, g.density)
             e0.record()
-            outs = tr.trace(*args, 0, sc.sph_degree, sc.min_transmittance)
+            outs = tr.trace(*args, 0, sc.sph_degree, sc.min_transmittance, tape=tape)
             e1.record()
-            tr.trace_bwd(*args[:4], *outs[:7], *args[4:], *grads, 0, sc.sph_degree, sc.min_transmittance)
+            tr.trace_bwd(*args[:4], *outs[:7], *args[4:], *grads, 0, sc.sph_degree, sc.min_transmittance, tape=tape)
             e2.record()
+            tr.trace_bwd(*args[:4], *outs[:7], *args[4:], *grads, 0, sc.sph_degree, sc.min_transmittance)
+            e3.record()
             if i >= warmup:
                 ev.append((e0, e1, e2))
+                ev_re.append((e2, e3))
         torch.cuda.synchronize()
         fwd = sum(a.elapsed_time(b) for a, b, _ in ev) / steps
         bwd = sum(b.elapsed_time(c) for _, b, c in ev) / steps
-        out[builder] = {"ms_fwd": round(fwd, 3), "ms_bwd": round(bwd, 3),
+        bwd_re = sum(a.elapsed_time(b) for a, b in ev_re) / steps
+        redo = float((tape[16:16 + sc.num_paths] < 0).float().mean())
+        out[builder] = {"ms_fwd": round(fwd, 3), "ms_bwd": round(bwd, 3), "ms_bwd_retrace": round(bwd_re, 3),
+                        "tape_retraced_pixels": round(redo, 4),
                         "rays_per_s_fwd_bwd": round(sc.num_paths / ((fwd + bwd) * 1e-3), 1)}
-        del tr, outs, grads
+        del tr, outs, grads, tape
     torch.cuda.empty_cache()
     return out
 

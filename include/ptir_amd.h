@@ -307,6 +307,10 @@ typedef struct ptir_grt_outputs {
     float* shading_normal;  /* [..,3] */
     float* hits_count;      /* [..,1] */
     float* visibility;      /* [N] */
+    uint32_t* hit_tape;     /* optional (NULL = none): ptir_grt_tape_bytes(B,H,W,hit_tape_cap) bytes
+                               of device memory that receive each pixel's accepted hits, for
+                               ptir_grt_trace_bwd to read instead of re-tracing */
+    uint32_t hit_tape_cap;  /* hits per pixel the tape holds, [1, 65535] */
 } ptir_grt_outputs;
 
 /* threedgrt OptixTracer::trace (threedgrt_tracer/src/optixTracer.cpp:855-925;
@@ -320,6 +324,10 @@ typedef struct ptir_grt_bwd_inputs {
     const float *radiance, *density, *hit_distance, *hit_distance2, *distortion, *shading_normal;
     const float *radiance_grad, *density_grad, *hit_distance_grad, *hit_distance2_grad, *distortion_grad,
         *shading_normal_grad;
+    const uint32_t* hit_tape;  /* the hit tape the forward wrote, or NULL: every pixel re-traces.  A
+                                  tape of another launch shape, frame or Gaussian count is ignored;
+                                  pixels whose hits overflowed it re-trace. */
+    uint32_t hit_tape_cap;     /* as given to the forward */
 } ptir_grt_bwd_inputs;
 
 /* Gradients, zeroed by the call (the reference allocates them with torch::zeros):
@@ -344,6 +352,11 @@ ptir_status ptir_grt_trace_bwd(ptir_tracer* tracer, const ptir_grt_scene* scene,
  * reference: it lets ptir_trace_bwd skip re-tracing (a 64 B record per path and per
  * bounce, plus 8 B per hit slot). */
 uint64_t ptir_tape_bytes(int32_t B, int32_t H, int32_t W, uint32_t max_bounces, uint32_t hits_per_path);
+
+/* Bytes of a 3DGRT hit tape for a [B,H,W] launch holding hits_per_pixel accepted hits per
+ * pixel (0 on invalid sizes): 64 B, 4 B per pixel, 8 B per hit slot.  No counterpart in the
+ * reference: it lets ptir_grt_trace_bwd skip re-tracing. */
+uint64_t ptir_grt_tape_bytes(int32_t B, int32_t H, int32_t W, uint32_t hits_per_pixel);
 
 /* ------------------------------------------------------------------------------------
  * Neural radiance cache.  No counterpart in the reference (its tiny-cuda-nn submodule is

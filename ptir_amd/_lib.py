@@ -23,7 +23,7 @@ PTIR_STATUS = {
 }
 
 EXPORTED = ["ptir_create", "ptir_destroy", "ptir_build_bvh", "ptir_check", "ptir_hierarchy_info", "ptir_grt_trace", "ptir_grt_trace_bwd",
-            "ptir_trace", "ptir_trace_bwd", "ptir_tape_bytes",
+            "ptir_trace", "ptir_trace_bwd", "ptir_tape_bytes", "ptir_grt_tape_bytes",
             "ptir_last_error", "ptir_version", "ptir_cache_create", "ptir_cache_destroy", "ptir_cache_num_weights",
             "ptir_cache_table_floats", "ptir_cache_read_params", "ptir_cache_write_params", "ptir_cache_read_grads",
             "ptir_cache_query", "ptir_cache_backprop", "ptir_cache_step", "ptir_cache_train", "ptir_cache_last_error",
@@ -108,13 +108,15 @@ class GrtScene(C.Structure):
 
 class GrtOutputs(C.Structure):
     _fields_ = [(k, C.c_void_p) for k in ("radiance", "density", "hit_distance", "hit_distance2", "distortion",
-                                           "normal", "shading_normal", "hits_count", "visibility")]
+                                           "normal", "shading_normal", "hits_count", "visibility", "hit_tape")] + \
+        [("hit_tape_cap", C.c_uint32)]
 
 
 class GrtBwdInputs(C.Structure):
     _fields_ = [(k, C.c_void_p) for k in ("radiance", "density", "hit_distance", "hit_distance2", "distortion",
                                            "shading_normal", "radiance_grad", "density_grad", "hit_distance_grad",
-                                           "hit_distance2_grad", "distortion_grad", "shading_normal_grad")]
+                                           "hit_distance2_grad", "distortion_grad", "shading_normal_grad",
+                                           "hit_tape")] + [("hit_tape_cap", C.c_uint32)]
 
 
 class GrtBwdOutputs(C.Structure):
@@ -155,6 +157,8 @@ def lib() -> C.CDLL:
                                          C.POINTER(GrtBwdOutputs), C.c_void_p]
         L.ptir_tape_bytes.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_uint32, C.c_uint32]
         L.ptir_tape_bytes.restype = C.c_uint64
+        L.ptir_grt_tape_bytes.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_uint32]
+        L.ptir_grt_tape_bytes.restype = C.c_uint64
         L.ptir_last_error.argtypes = [C.c_void_p]
         L.ptir_last_error.restype = C.c_char_p
         L.ptir_version.restype = C.c_char_p

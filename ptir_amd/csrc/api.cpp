@@ -526,9 +526,12 @@ ptir_status ptir_grt_trace(ptir_tracer* t, const ptir_grt_scene* g, const ptir_l
     const ptir_scene sc = grt_as_scene(g);
     ptir::DeviceScene S = make_scene(t, &sc, la);
     S.max_bounces = 1u;
+    if (out->hit_tape && (out->hit_tape_cap == 0 || out->hit_tape_cap > 0xFFFFu))
+        return fail(t, PTIR_ERR_INVALID_ARGUMENT, "hit_tape_cap must be in [1, 65535]");
     const ptir::grt::FwdOut o{out->radiance, out->density, out->hit_distance, out->hit_distance2, out->distortion,
                               out->normal, out->shading_normal, out->hits_count,
-                              reinterpret_cast<uint32_t*>(out->visibility)};
+                              reinterpret_cast<uint32_t*>(out->visibility), out->hit_tape,
+                              out->hit_tape ? out->hit_tape_cap : 0u};
     e = ptir::launch_grt_fwd(S, o, t->opts.kbuffer, t->short_stack, s);
     if (e != hipSuccess) return hip_fail(t, e, "primary-only trace launch");
     return PTIR_OK;
@@ -561,11 +564,13 @@ ptir_status ptir_grt_trace_bwd(ptir_tracer* t, const ptir_grt_scene* g, const pt
     const ptir_scene sc = grt_as_scene(g);
     ptir::DeviceScene S = make_scene(t, &sc, la);
     S.max_bounces = 1u;
-    const ptir::grt::BwdIO io{in->radiance, in->density, in->hit_distance, in->hit_distance2, in->distortion,
-                              in->shading_normal, in->radiance_grad, in->density_grad, in->hit_distance_grad,
-                              in->hit_distance2_grad, in->distortion_grad, in->shading_normal_grad,
-                              g->particle_density, g->shading_normal, out->density_grad, out->radiance_grad,
-                              g->shading_normal ? out->shading_normal_grad : nullptr};
+    ptir::grt::BwdIO io{in->radiance, in->density, in->hit_distance, in->hit_distance2, in->distortion,
+                        in->shading_normal, in->radiance_grad, in->density_grad, in->hit_distance_grad,
+                        in->hit_distance2_grad, in->distortion_grad, in->shading_normal_grad,
+                        g->particle_density, g->shading_normal, out->density_grad, out->radiance_grad,
+                        g->shading_normal ? out->shading_normal_grad : nullptr};
+    io.tape = in->hit_tape;
+    io.tape_cap = in->hit_tape ? in->hit_tape_cap : 0u;
     // Hit records (64 B per slot, 32 slots per pixel) replace most of the per-hit gradient
     // atomics; where they cannot be had (slot indices past 32 bits, or no memory for them),
     // every hit adds its gradients by atomics.
@@ -577,6 +582,11 @@ ptir_status ptir_grt_trace_bwd(ptir_tracer* t, const ptir_grt_scene* g, const pt
     e = ptir::launch_grt_bwd(S, io, t->opts.kbuffer, t->short_stack, rec, N, s);
     if (e != hipSuccess) return hip_fail(t, e, "primary-only backward launch");
     return PTIR_OK;
+}
+
+uint64_t ptir_grt_tape_bytes(int32_t B, int32_t H, int32_t W, uint32_t hits_per_pixel) {
+    if (B < 0 || H < 0 || W < 0 || hits_per_pixel == 0 || hits_per_pixel > 0xFFFFu) return 0;
+    return ptir::grt::tape_bytes((uint64_t)B * H * W, hits_per_pixel);
 }
 
 uint64_t ptir_tape_bytes(int32_t B, int32_t H, int32_t W, uint32_t max_bounces, uint32_t hits_per_path) {

@@ -209,9 +209,30 @@ struct AdjIO {
 // 3DGRT primary-only tracer (grt_path.hpp): forward outputs, and the backward's forward
 // outputs, output gradients, raw Gaussian parameters and gradient sinks.
 namespace grt {
+// Hit tape of the 3DGRT forward: per pixel, the accepted hits its backward processes, in
+// order, so that the backward need not walk the hierarchy again.  Words: a header of
+// kTapeHeader words, the per-pixel counts [P] (bit 31: the backward re-traces the pixel), the
+// leaf slots [cap * P] and the distances [cap * P] (entry j of pixel p at j * P + p).
+constexpr uint32_t kTapeHeader = 16;
+constexpr uint32_t kTapeMagic = 0x54545247u;  // "GRTT"
+constexpr uint32_t kTapeRetrace = 0x80000000u;
+struct TapeView {
+    uint32_t* base;
+    uint32_t cap;
+    size_t P;
+    __host__ __device__ uint32_t* header() const { return base; }
+    __host__ __device__ uint32_t* count() const { return base + kTapeHeader; }
+    __host__ __device__ uint32_t* slot() const { return base + kTapeHeader + P; }
+    __host__ __device__ float* dist() const { return reinterpret_cast<float*>(base + kTapeHeader + P + size_t(cap) * P); }
+};
+__host__ __device__ inline uint64_t tape_bytes(uint64_t P, uint32_t cap) {
+    return 4ull * (kTapeHeader + P + 2ull * cap * P);
+}
 struct FwdOut {
     float *rad, *dns, *hit, *hit2, *dist, *nrm, *snrm, *hits;  // [P,3], [P], [P,2], [P], [P], [P,3], [P,3], [P]
     uint32_t* vis;                                              // [N]
+    uint32_t* tape;                                             // hit tape, or null
+    uint32_t tape_cap;                                          // hits per pixel it holds
 };
 struct BwdIO {
     const float *rad, *dns, *hit, *hit2, *dist, *snrm;                               // forward outputs
@@ -226,6 +247,8 @@ struct BwdIO {
     uint32_t* rec_key;
     uint32_t* rec_cnt;
     uint32_t paths;
+    const uint32_t* tape;  // the forward's hit tape, or null (every pixel re-traces)
+    uint32_t tape_cap;
 };
 // Records per pixel of the 3DGRT backward; hits beyond them add their gradients by atomics.
 constexpr uint32_t kRecPerPixel = 32;

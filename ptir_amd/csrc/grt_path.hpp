@@ -26,6 +26,14 @@ __device__ __forceinline__ void fwd_path(const DeviceScene& S, const FwdOut& out
     const float dlen = sqrtf(dot(ray.d, ray.d));
     count(S, C_PATHS);
     count(S, C_SEGMENTS);
+    const size_t P = size_t(S.B) * S.H * S.W;
+    const TapeView tape{out.tape, out.tape_cap, P};
+    uint32_t ntape = 0;  // accepted hits (the tape holds the first tape_cap)
+    if (out.tape && pix == 0) {
+        uint32_t* h = tape.header();
+        h[0] = kTapeMagic; h[1] = uint32_t(S.B); h[2] = uint32_t(S.H); h[3] = uint32_t(S.W);
+        h[4] = S.frame; h[5] = S.n; h[6] = out.tape_cap; h[7] = 0u;
+    }
     // gaussianParticles.cuh:329-410 processHit
     auto proc = [&](float t, int slot) {
         count(S, C_CANDIDATES);
@@ -50,6 +58,13 @@ __device__ __forceinline__ void fwd_path(const DeviceScene& S, const FwdOut& out
             if (out.vis[id] == 0u) out.vis[id] = 1u;
             count(S, C_ACCEPTED);
             count(S, C_ACCEPTED_PRIMARY);
+            if (out.tape) {
+                if (ntape < out.tape_cap) {
+                    tape.slot()[size_t(ntape) * P + pix] = uint32_t(slot);
+                    tape.dist()[size_t(ntape) * P + pix] = t;
+                }
+                ++ntape;
+            }
         }
         lastHit = fmaxf(lastHit, t);
         if (S.enable_hitcounts) hits += a > 0.f ? 1.f : 0.f;
@@ -78,6 +93,19 @@ __device__ __forceinline__ void fwd_path(const DeviceScene& S, const FwdOut& out
                 proc(__uint_as_float(L.at(k)), int(L.at(K + k)));
             }
         }
+    }
+    if (out.tape) {
+        // The backward processes the candidates in (its start, min(lastHit, t1) + eps): the
+        // accepted hits at or past that end leave the tape.  Where the transmittance ended
+        // the walk and the end lies past lastHit, candidates the forward never reached may
+        // lie below it: the backward re-traces that pixel, as it does one whose hits
+        // overflow the tape.
+        const float endT = fminf(lastHit, t1) + kEpsT;
+        uint32_t n = ntape;
+        bool redo = n > out.tape_cap || (!(T > minT) && endT > lastHit);
+        if (!redo)
+            while (n > 0 && tape.dist()[size_t(n - 1) * P + pix] >= endT) --n;
+        tape.count()[pix] = redo ? kTapeRetrace : n;
     }
     float* o = out.rad + 3 * pix;
     o[0] = rad.x; o[1] = rad.y; o[2] = rad.z;
@@ -332,10 +360,23 @@ __device__ __forceinline__ void bwd_path(const DeviceScene& S, const BwdIO& io, 
     const float endT = fminf(maxHit, t1) + kEpsT;
     count(S, C_PATHS);
     uint32_t nrec = 0;
+    // the forward's hit tape, when it is this launch's and holds the pixel's hits
+    bool taped = false;
+    if (io.tape) {
+        const size_t P = size_t(S.B) * S.H * S.W;
+        const TapeView tape{const_cast<uint32_t*>(io.tape), io.tape_cap, P};
+        const uint32_t* h = tape.header();
+        const bool mine = h[0] == kTapeMagic && h[1] == uint32_t(S.B) && h[2] == uint32_t(S.H) &&
+                          h[3] == uint32_t(S.W) && h[4] == S.frame && h[5] == S.n && h[6] == io.tape_cap;
+        const uint32_t n = mine ? tape.count()[pix] : kTapeRetrace;
+        taped = (n & kTapeRetrace) == 0u;
+        if (taped)
+            for (uint32_t j = 0; j < n; ++j) hit_bwd_pix(S, io, ray, int(tape.slot()[size_t(j) * P + pix]), b, pix, nrec);
+    }
     if (wstack) {
         bool done = false;
         for (;;) {
-            const bool need = !done && startT < endT;
+            const bool need = !taped && !done && startT < endT;
             if (__ballot(need) == 0) break;
             const int cnt = wave_collect<K, SD>(S, ray, startT + kEpsT, endT, need, L, wstack);
             if (need) {
@@ -347,7 +388,7 @@ __device__ __forceinline__ void bwd_path(const DeviceScene& S, const BwdIO& io, 
                 }
             }
         }
-    } else {
+    } else if (!taped) {
         while (startT < endT) {
             const int cnt = collect<K, SD>(S, ray, startT + kEpsT, endT, L);
             if (cnt == 0) break;

@@ -60,10 +60,23 @@ class GRTTracer:
             sc.shading_normal = _ptr(sn)
         return sc, keep
 
+    def new_tape(self, ray_dir, hits_per_pixel: int = 64) -> torch.Tensor:
+        """A hit tape for launches of ray_dir's [B,H,W] shape: passed as `tape=` to trace, it
+        receives each pixel's accepted hits (up to hits_per_pixel); passed to trace_bwd, the
+        backward reads them instead of re-tracing (pixels with more hits re-trace)."""
+        B, H, W = ray_dir.shape[:3]
+        n = _lib.lib().ptir_grt_tape_bytes(B, H, W, hits_per_pixel)
+        if n == 0:
+            raise ValueError("invalid tape size")
+        t = torch.empty((n + 3) // 4, dtype=torch.int32, device=ray_dir.device)
+        t.hits_per_pixel = hits_per_pixel
+        return t
+
     def trace(self, frame_id, ray_to_world, ray_ori, ray_dir, particle_density, particle_radiance,
-              particle_shading_normal, render_opts=0, sph_degree=3, min_transmittance=1e-3):
+              particle_shading_normal, render_opts=0, sph_degree=3, min_transmittance=1e-3, *, tape=None):
         """OptixTracer::trace (threedgrt_tracer/src/optixTracer.cpp:855-925): rad, dns,
-        hit[2], hit2, distortion, nrm, snrm, hits, visibility."""
+        hit[2], hit2, distortion, nrm, snrm, hits, visibility.  tape: a new_tape() to record
+        the hits in (no counterpart in the reference)."""
         sc, keep = self._scene(particle_density, particle_radiance, particle_shading_normal)
         la, keep2 = HipTracer._launch(frame_id, ray_to_world, ray_ori, ray_dir, render_opts, sph_degree,
                                       min_transmittance, 1)
@@ -79,6 +92,10 @@ class GRTTracer:
         hits = torch.empty(B, H, W, 1, **f)
         vis = torch.zeros(sc.num_gaussians, 1, **f)
         out = _lib.GrtOutputs(*[_ptr(t) for t in (rad, dns, hit, hit2, dist, nrm, snrm, hits, vis)])
+        if tape is not None:
+            if tape.numel() * 4 < _lib.lib().ptir_grt_tape_bytes(B, H, W, tape.hits_per_pixel):
+                raise ValueError("tape is too small for this launch")
+            out.hit_tape, out.hit_tape_cap = _ptr(tape), tape.hits_per_pixel
         _lib.check(_lib.lib().ptir_grt_trace(self._t._h, C.byref(sc), C.byref(la), C.byref(out), _stream()),
                    self._t._h)
         del keep, keep2
@@ -87,7 +104,7 @@ class GRTTracer:
     def trace_bwd(self, frame_id, ray_to_world, ray_ori, ray_dir, ray_rad, ray_dns, ray_hit, ray_hit2, ray_dist,
                   ray_nrm, ray_snrm, particle_density, particle_radiance, particle_shading_normal, ray_rad_grd,
                   ray_dns_grd, ray_hit_grd, ray_hit2_grd, ray_dist_grd, ray_nrm_grd, ray_snrm_grd, render_opts=0,
-                  sph_degree=3, min_transmittance=1e-3):
+                  sph_degree=3, min_transmittance=1e-3, *, tape=None):
         """OptixTracer::traceBwd (threedgrt_tracer/src/optixTracer.cpp:927-1012): returns
         (particle_density_grd [N,12], particle_radiance_grd, particle_shading_normal_grd).
         ray_hit_grd is the gradient of the integrated depth ([..,1], channel 0 of ray_hit);
@@ -114,6 +131,10 @@ class GRTTracer:
                pix(ray_hit2_grd, "ray_hit2_grd", 1, True), pix(ray_dist_grd, "ray_dist_grd", 1, True),
                pix(ray_snrm_grd, "ray_snrm_grd", 3, True)]
         bi = _lib.GrtBwdInputs(*[_ptr(t) for t in ins])
+        if tape is not None:
+            if tape.numel() * 4 < _lib.lib().ptir_grt_tape_bytes(B, H, W, tape.hits_per_pixel):
+                raise ValueError("tape is too small for this launch")
+            bi.hit_tape, bi.hit_tape_cap = _ptr(tape), tape.hits_per_pixel
         n = sc.num_gaussians
         pd = keep[0]
         f = dict(dtype=torch.float32, device=pd.device)
@@ -136,14 +157,17 @@ class Tracer:
         def forward(ctx, tracer_wrapper, frame_id, ray_to_world, ray_ori, ray_dir, mog_pos, mog_rot, mog_scl, mog_dns,
                     mog_sph, mog_snrm, render_opts, sph_degree, min_transmittance):
             particle_density = torch.concat([mog_pos, mog_dns, mog_rot, mog_scl, torch.zeros_like(mog_dns)], dim=1)
+            # the backward reads this forward's hits from a tape instead of re-tracing them
+            tape = tracer_wrapper.new_tape(ray_dir) if any(ctx.needs_input_grad[5:11]) else None
             outs = tracer_wrapper.trace(frame_id, ray_to_world, ray_ori, ray_dir, particle_density, mog_sph, mog_snrm,
-                                        render_opts, sph_degree, min_transmittance)
+                                        render_opts, sph_degree, min_transmittance, tape=tape)
             rad, dns, hit, hit2, dist, nrm, snrm, hits, vis = outs
             ctx.save_for_backward(ray_to_world, ray_ori, ray_dir, rad, dns, hit, hit2, dist, nrm, snrm,
                                   particle_density, mog_sph, mog_snrm)
             ctx.frame_id, ctx.render_opts = frame_id, render_opts
             ctx.sph_degree, ctx.min_transmittance = sph_degree, min_transmittance
             ctx.tracer_wrapper = tracer_wrapper
+            ctx.tape = tape
             return rad, dns, hit[..., 0:1], hit2, dist, nrm, snrm, hits, vis
 
         @staticmethod
@@ -154,7 +178,7 @@ class Tracer:
                 ctx.frame_id, ray_to_world, ray_ori, ray_dir, rad, dns, hit, hit2, dist, nrm, snrm, particle_density,
                 mog_sph, mog_snrm, rad_grd.contiguous(), dns_grd.contiguous(), hit_grd.contiguous(),
                 hit2_grd.contiguous(), dist_grd.contiguous(), nrm_grd.contiguous(), snrm_grd.contiguous(),
-                ctx.render_opts, ctx.sph_degree, ctx.min_transmittance)
+                ctx.render_opts, ctx.sph_degree, ctx.min_transmittance, tape=ctx.tape)
             pos_g, dns_g, rot_g, scl_g, _ = torch.split(d_dns, [3, 1, 4, 3, 1], dim=1)
             return (None, None, None, None, None, pos_g, rot_g, scl_g, dns_g, d_sph, d_sn, None, None, None)
 

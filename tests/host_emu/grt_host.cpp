@@ -3,6 +3,9 @@
 // host_scene.hpp: the forward writes f_rad, f_dns, f_hit, f_hit2, f_dist, f_nrm, f_snrm,
 // f_hits, f_vis (.f32); then the backward, fed with those outputs and the gradients g_rad,
 // g_dns, g_hit (depth), g_hit2, g_dist, g_snrm (.f32), writes d_density, d_sh, d_snormal.
+// The forward also fills a hit tape of meta tape_cap hits per pixel; a second backward reads
+// it instead of re-tracing and writes t_density, t_sh, t_snormal, and tape_redo.f32 holds the
+// share of pixels whose tape said re-trace.
 #include "../../ptir_amd/csrc/grt_path.hpp"
 #include "host_scene.hpp"
 
@@ -19,8 +22,10 @@ int main(int argc, char** argv) {
     const int N = la.N;
     std::vector<float> rad(3 * P), dns(P), hit(2 * P), hit2(P), dist(P), nrm(3 * P), snrm(3 * P), hits(P);
     std::vector<uint32_t> vis(size_t(N), 0u);
+    const uint32_t cap = uint32_t(la.m.get("tape_cap", 8));
+    std::vector<uint32_t> tape(size_t((grt::tape_bytes(P, cap) + 3) / 4), 0u);
     const grt::FwdOut fo{rad.data(), dns.data(), hit.data(), hit2.data(), dist.data(), nrm.data(), snrm.data(),
-                         hits.data(), vis.data()};
+                         hits.data(), vis.data(), tape.data(), cap};
     la.S.wave_traversal = int(la.m.get("wave_traversal", 0));
     std::vector<uint32_t> lds(size_t(LdsLayout<16, kStackDeep>::kWords) * kTraceBlock, 0u);
     std::vector<uint32_t> wst(kStackDeep, 0u);  // one-lane waves: one wave stack
@@ -54,6 +59,23 @@ int main(int argc, char** argv) {
     host::write_f32("d_density", d_density);
     host::write_f32("d_sh", d_sh);
     host::write_f32("d_snormal", d_snormal);
+    std::vector<float> t_density(size_t(N) * 12, 0.f), t_sh(size_t(N) * la.shs, 0.f), t_snormal(size_t(N) * 3, 0.f);
+    grt::BwdIO iot = io;
+    iot.d_density = t_density.data();
+    iot.d_sh = t_sh.data();
+    iot.d_snormal = t_snormal.data();
+    iot.tape = tape.data();
+    iot.tape_cap = cap;
+    la.for_each_path([&](const PixelId& px, int t) {
+        const Lane L{lds.data() + t};
+        grt::bwd_path<16, kStackDeep>(la.S, iot, L, la.S.wave_traversal ? wst.data() : nullptr, px.pix);
+    });
+    host::write_f32("t_density", t_density);
+    host::write_f32("t_sh", t_sh);
+    host::write_f32("t_snormal", t_snormal);
+    size_t redo = 0;
+    for (size_t i = 0; i < P; ++i) redo += (tape[grt::kTapeHeader + i] & grt::kTapeRetrace) ? 1 : 0;
+    host::write_f32("tape_redo", std::vector<float>{float(redo) / float(P)});
     printf("%d blocks, traversal stack overflow flag %d\n", la.S.num_blocks, la.error_flag);
     return la.error_flag;
 }

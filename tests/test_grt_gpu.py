@@ -86,6 +86,25 @@ def test_grt_forward_and_backward_match_the_oracle(device, seed, dense):
         print(f"grt seed {seed} {name}: relative L2 {err:.2e}", flush=True)
         assert np.isfinite(a).all() and err < 2e-3, name
 
+    # the backward that reads the forward's hit tape (16 hits per pixel: pixels with more
+    # re-trace) gives the re-tracing backward's gradients (to the float rounding of the
+    # atomics' order) and the same forward outputs
+    tape = tr.new_tape(args[3], hits_per_pixel=16)
+    outs_t = tr.trace(*args, 0, sc.sph_degree, sc.min_transmittance, tape=tape)
+    for a, b in zip(outs, outs_t):
+        assert torch.equal(a, b)
+    td, tsh, tsn = tr.trace_bwd(*args[:4], *outs_t[:7], *args[4:], t["rad"], t["dns"], t["hit"], t["hit2"],
+                                t["distortion"], torch.zeros_like(outs[5]), t["snrm"], 0, sc.sph_degree,
+                                sc.min_transmittance, tape=tape)
+    torch.cuda.synchronize()
+    redo = float((tape[16:16 + B * H * W] < 0).float().mean())  # count words with bit 31 set
+    print(f"grt seed {seed}: {redo:.3f} of pixels re-traced with the tape", flush=True)
+    for name, a, b in (("density", td, hd), ("sh", tsh, hsh), ("snormal", tsn, hsn)):
+        a = a.cpu().double().numpy()
+        err = rel_l2(a, b)
+        print(f"grt seed {seed} {name}: tape vs re-trace relative L2 {err:.2e}", flush=True)
+        assert err < 1e-5, name
+
 
 def test_grt_outputs_do_not_depend_on_builder_or_traversal(device):
     sc = _scene(4)

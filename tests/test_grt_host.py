@@ -74,3 +74,11 @@ def test_grt_host_build_matches_oracle(harness, tmp_path, seed, wave):
         err = rel_l2(a, b)
         print(f"seed {seed} {name}: relative L2 {err:.2e}", flush=True)
         assert np.isfinite(a).all() and err < 2e-3, name
+    # the backward that reads the forward's hit tape (8 hits per pixel: the pixels with more
+    # re-trace) adds the same gradients in the same order as the one that re-traces
+    redo = float(rd("tape_redo", (1,))[0])
+    print(f"seed {seed}: {redo:.3f} of pixels re-traced", flush=True)
+    assert 0.0 < redo < 1.0
+    assert np.array_equal(rd("t_density", (N, 12)), hd)
+    assert np.array_equal(rd("t_sh", tuple(sc.features.shape)), hsh)
+    assert np.array_equal(rd("t_snormal", (N, 3)), hsn)
