@@ -227,7 +227,9 @@ __device__ __forceinline__ bool hit_bwd(const DeviceScene& S, const BwdIO& io, c
     const float* c = S.sh + size_t(S.sh_stride) * id;
     const int ncoef = (S.sph_degree + 1) * (S.sph_degree + 1);
     V3 raw = v3s(0.5f);
-    for (int k = 0; k < ncoef; ++k) raw += sh_basis(k, ray.d) * v3(c[3 * k], c[3 * k + 1], c[3 * k + 2]);
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+        if (k < ncoef) raw += sh_basis(k, ray.d) * v3(c[3 * k], c[3 * k + 1], c[3 * k + 2]);
     const V3 col = v3(fmaxf(raw.x, 0.f), fmaxf(raw.y, 0.f), fmaxf(raw.z, 0.f));
     const V3 dRGB = v3(raw.x > 0.f ? b.radG.x * weight : 0.f, raw.y > 0.f ? b.radG.y * weight : 0.f,
                        raw.z > 0.f ? b.radG.z * weight : 0.f);
