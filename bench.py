@@ -163,7 +163,7 @@ def cpu_baseline(scene, table, stride: int) -> dict:
 def time_primary_only(steps: int = 10, warmup: int = 2) -> dict:
     """The 3DGRT primary-only tracer on the object-scale rays: G-buffer forward and the
     backward to every Gaussian parameter (random output gradients), timed with HIP events.
-    The forward records its hits on a tape (64 per pixel) that the backward reads, as the
+    The forward records its hits on a tape (128 per pixel) that the backward reads, as the
     autograd Tracer does; `ms_bwd_retrace` is the backward that re-traces every pixel."""
     import torch
 
@@ -179,7 +179,7 @@ def time_primary_only(steps: int = 10, warmup: int = 2) -> dict:
         args = (0, g.ray_to_world, g.rays_ori, g.rays_dir, g.particle_density(), g.features, g.shading_normal)
         gen = torch.Generator(device=dev).manual_seed(0)
         tr.build_bvh(g.positions, g.rotation, g.scale, g.density)
-        tape = tr.new_tape(g.rays_dir, hits_per_pixel=64)
+        tape = tr.new_tape(g.rays_dir, hits_per_pixel=128)
         outs = tr.trace(*args, 0, sc.sph_degree, sc.min_transmittance)
         grads = [torch.randn(o.shape, device=dev, generator=gen) for o in outs[:7]]
         grads[2] = grads[2][..., :1].contiguous()

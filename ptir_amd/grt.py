@@ -60,10 +60,11 @@ class GRTTracer:
             sc.shading_normal = _ptr(sn)
         return sc, keep
 
-    def new_tape(self, ray_dir, hits_per_pixel: int = 64) -> torch.Tensor:
+    def new_tape(self, ray_dir, hits_per_pixel: int = 128) -> torch.Tensor:
         """A hit tape for launches of ray_dir's [B,H,W] shape: passed as `tape=` to trace, it
         receives each pixel's accepted hits (up to hits_per_pixel); passed to trace_bwd, the
-        backward reads them instead of re-tracing (pixels with more hits re-trace)."""
+        backward reads them instead of re-tracing (pixels with more hits re-trace, and each
+        such pixel costs its whole wave a hierarchy walk, so the default is generous)."""
         B, H, W = ray_dir.shape[:3]
         n = _lib.lib().ptir_grt_tape_bytes(B, H, W, hits_per_pixel)
         if n == 0:
