@@ -64,9 +64,9 @@ __global__ void __launch_bounds__(256) k_grt_scatter(const uint32_t* __restrict_
 __global__ void __launch_bounds__(256) k_grt_sum(DeviceScene S, grt::BwdIO io, const uint32_t* __restrict__ gcount,
                                                  const uint32_t* __restrict__ goff, const uint32_t* __restrict__ bucket,
                                                  uint32_t n) {
-    const uint32_t id = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t id = blockIdx.x * 64u + threadIdx.x / 4u;  // four lanes per Gaussian
     if (id >= n) return;
-    grt::sum_records(S, io, id, bucket + goff[id], gcount[id]);
+    grt::sum_records(S, io, id, bucket + goff[id], gcount[id], threadIdx.x & 3u);
 }
 
 hipError_t grt_rec_reserve(GrtRecBuffers& r, size_t paths, size_t n) {
@@ -166,7 +166,7 @@ hipError_t launch_grt_bwd(const DeviceScene& S0, const grt::BwdIO& io0, int kbuf
     }
     if ((e = dispatch(kbuf, short_stack, GrtBwdLaunch{S, io, s})) != hipSuccess) return e;
     if (!records) return hipSuccess;
-    const unsigned pb = unsigned((paths + 255) / 256), nb = unsigned((n + 255) / 256);
+    const unsigned pb = unsigned((paths + 255) / 256), nb = unsigned((n + 63) / 64);
     if ((e = fill_zero(rec->gcount, n * sizeof(uint32_t), s)) != hipSuccess) return e;
     if ((e = fill_zero(rec->gcursor, n * sizeof(uint32_t), s)) != hipSuccess) return e;
     k_grt_hist<<<pb, 256, 0, s>>>(rec->key, rec->cnt, uint32_t(paths), rec->gcount);

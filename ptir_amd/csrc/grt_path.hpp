@@ -405,17 +405,18 @@ __device__ __forceinline__ void bwd_path(const DeviceScene& S, const BwdIO& io, 
 }
 
 // k_grt_sum for Gaussian id: the gradients of its n records (their slots listed at
-// bucket[0, n)), added to what the hits beyond the records left by atomics.  One lane per
-// Gaussian, so no other lane writes its gradients.
+// bucket[0, n)), added to what the hits beyond the records left by atomics.  Four lanes per
+// Gaussian (q = 0..3 takes records q, q + 4, ...), their sums combined in the quad; lane 0
+// writes, so no other lane writes the Gaussian's gradients.
 __device__ __forceinline__ void sum_records(const DeviceScene& S, const BwdIO& io, uint32_t id, const uint32_t* bucket,
-                                            uint32_t n) {
+                                            uint32_t n, uint32_t q) {
     const int ncoef = (S.sph_degree + 1) * (S.sph_degree + 1);
     float sh[48];
 #pragma unroll
     for (int i = 0; i < 48; ++i) sh[i] = 0.f;
     V3 dn = v3s(0.f), gpos = v3s(0.f), gscl = v3s(0.f);
     float gdens = 0.f, dq[4] = {0.f, 0.f, 0.f, 0.f};
-    for (uint32_t i = 0; i < n; ++i) {
+    for (uint32_t i = q; i < n; i += 4) {
         const size_t s = bucket[i];
         const size_t pix = s % io.paths;
         const float4* R = io.rec + 4 * s;
@@ -441,6 +442,19 @@ __device__ __forceinline__ void sum_records(const DeviceScene& S, const BwdIO& i
         gscl += v3(r3.x, r3.y, r3.z);
     }
     if (n == 0) return;
+    // the quad's four partial sums (lanes q = 0..3 of one Gaussian)
+    auto quad = [](float v) {
+        v += __shfl_xor(v, 1);
+        return v + __shfl_xor(v, 2);
+    };
+#pragma unroll
+    for (int i = 0; i < 48; ++i) sh[i] = quad(sh[i]);
+    dn = v3(quad(dn.x), quad(dn.y), quad(dn.z));
+    gpos = v3(quad(gpos.x), quad(gpos.y), quad(gpos.z));
+    gscl = v3(quad(gscl.x), quad(gscl.y), quad(gscl.z));
+    gdens = quad(gdens);
+    for (int i = 0; i < 4; ++i) dq[i] = quad(dq[i]);
+    if (q != 0) return;
     float* dsh = io.d_sh + size_t(S.sh_stride) * id;
 #pragma unroll
     for (int k = 0; k < 16; ++k) {

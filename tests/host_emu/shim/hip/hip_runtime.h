@@ -34,6 +34,8 @@ inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
 inline int __popc(unsigned x) { return __builtin_popcount(x); }
 inline unsigned __lane_id() { return 0; }
 template <class T> inline T __shfl(T v, int) { return v; }
+// one-lane waves: no other lane holds a value to exchange
+template <class T> inline T __shfl_xor(T, int) { return T(0); }
 inline float __uint_as_float(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
 inline float __int_as_float(int32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
 inline uint32_t __float_as_uint(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
